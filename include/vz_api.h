@@ -1,0 +1,211 @@
+/*
+ * vz_api.h -- C ABI of the MI355X-native sphere path tracer (libvz.so).
+ *
+ * This is the drop-in boundary for the OptiX launch of TienYoung/Venusaur.
+ * Plain C: POD structs, plain pointers and sizes, status codes.  Streams are
+ * passed as `vzStream` (a hipStream_t cast to void*; NULL = the null stream).
+ * Every call that enqueues GPU work is stream-ordered; calls that return data
+ * to the host synchronise the stream they were given.
+ *
+ * Reference interfaces replaced (file:line in the reference tree):
+ *   Renderer::Init / CreateContext ........ Core/Renderer.h:25-33, 144-158
+ *   Renderer::BuildAccelerationStructures . Core/Renderer.h:160-255 (optixAccelBuild)
+ *   Renderer::CreateSBT ................... Core/Renderer.h:452-520 (per-sphere records)
+ *   Renderer::Draw ........................ Core/Renderer.h:35-78   (optixLaunch, line 75)
+ *   Renderer::Cleanup ..................... Core/Renderer.h:80-97
+ *   Params / SphereHitGroupData / MaterialData  Core/RayTracer.h:3-45
+ *   optixTrace (closest hit over the GAS) . Core/RayTracer.cu:190-202
+ *   Scene::Scene (book scene) ............. Core/Scene.h:13-112
+ *   Camera::UVWFrame ...................... Core/Camera.cpp:16-37
+ */
+#ifndef VZ_API_H
+#define VZ_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VZ_API_VERSION 1
+
+typedef enum vzStatus {
+    VZ_SUCCESS = 0,
+    VZ_ERROR_INVALID_VALUE = 1,  /* bad argument (null pointer, size, range) */
+    VZ_ERROR_OUT_OF_MEMORY = 2,  /* device or host allocation failed */
+    VZ_ERROR_HIP = 3,            /* a HIP runtime call failed */
+    VZ_ERROR_NOT_READY = 4,      /* serving: result not finished yet */
+    VZ_ERROR_CAPACITY = 5,       /* request does not fit (HBM budget, queue, table) */
+    VZ_ERROR_UNSUPPORTED = 6,    /* feature not available (e.g. GL interop) */
+    VZ_ERROR_INTERNAL = 7,       /* internal invariant violated */
+    VZ_ERROR_NO_DEVICE = 8       /* no usable gfx950 device */
+} vzStatus;
+
+typedef void* vzStream;                    /* hipStream_t */
+typedef struct vzContext_st* vzContext;    /* device context (Renderer State, Renderer.h:111-132) */
+typedef struct vzScene_st* vzScene;        /* resident scene + BVH (the GAS + SBT) */
+typedef struct vzServer_st* vzServer;      /* serving loop */
+
+enum { VZ_LAMBERTIAN = 0, VZ_METAL = 1, VZ_DIELECTRIC = 2 }; /* material.h:9-14 */
+
+/* SphereHitGroupData {center, radius} (RayTracer.h:40-45) */
+typedef struct vzSphere { float center[3]; float radius; } vzSphere;
+/* Material (material.h:16-32) / MaterialData (RayTracer.h:27-38), unpacked */
+typedef struct vzMaterial { uint32_t type; float albedo[3]; float fuzz; float ir; } vzMaterial;
+/* One closest-hit query, the arguments of optixTrace (RayTracer.cu:190-202) */
+typedef struct vzRay { float origin[3]; float tmin; float dir[3]; float tmax; } vzRay;
+/* Closest hit: t of the hit (tmax on a miss) and the primitive id
+ * (index into the sphere array given at scene creation), VZ_MISS on a miss. */
+typedef struct vzHit { float t; uint32_t prim; } vzHit;
+#define VZ_MISS 0xFFFFFFFFu
+
+/* Launch flags */
+#define VZ_ACCUM_FP64 0x1u     /* accum is double4 (required above 1024 spp) */
+#define VZ_MEGAKERNEL 0x2u     /* one-thread-per-pixel megakernel instead of the wavefront */
+#define VZ_COUNT_VISITS 0x4u   /* count BVH node fetches and sphere tests into vzStats */
+#define VZ_TIME_STAGES 0x8u    /* time every trace-kernel launch with HIP events */
+
+/* Params (RayTracer.h:3-17).  image/accum are device pointers.  The
+ * traversable handle is replaced by the vzScene argument of vzLaunch. */
+typedef struct vzLaunchParams {
+    void* image;                 /* uchar4[width*height] or NULL */
+    void* accum;                 /* float4[width*height] (double4 with VZ_ACCUM_FP64), required */
+    uint32_t width, height;      /* >= 2 each (RayTracer.cu:173-174 divides by w-1, h-1) */
+    uint32_t samples_per_pixel;  /* >= 1 */
+    uint32_t subframe_index;     /* frame number since reset, >= 1; seeds tea<4>(pixel, frame) */
+    float origin[3], u[3], v[3], w[3];
+    float lens_radius;
+    uint32_t max_depth;          /* segments per path, 1..255 (the reference hard-codes 4) */
+    uint32_t flags;              /* VZ_* flags above */
+} vzLaunchParams;
+
+typedef struct vzStats {
+    uint64_t segments;       /* closest-hit queries traced (the metric's unit) */
+    uint64_t paths;          /* camera paths started */
+    uint64_t node_visits;    /* BVH node fetches (VZ_COUNT_VISITS only) */
+    uint64_t sphere_tests;   /* leaf sphere tests (VZ_COUNT_VISITS only) */
+    double render_ms;        /* wall time of the launch (HIP events) */
+    double trace_ms;         /* sum of trace-kernel times (VZ_TIME_STAGES only) */
+    uint32_t trace_launches; /* trace-kernel launches that had work */
+    uint32_t iterations;     /* wavefront iterations */
+} vzStats;
+
+typedef struct vzAccelInfo {
+    uint32_t num_prims;
+    uint32_t num_nodes;       /* internal nodes, num_prims - 1 */
+    uint32_t depth;           /* levels from the root to the deepest leaf */
+    uint32_t node_bytes;      /* bytes per node (32) */
+    uint64_t device_bytes;    /* resident scene + BVH footprint */
+    double build_ms;          /* last build/refit time (HIP events) */
+} vzAccelInfo;
+
+/* ---- context -------------------------------------------------------- */
+vzStatus vzCreate(int device, vzContext* out);
+vzStatus vzDestroy(vzContext ctx);
+/* Device properties used by the autotuner/admission: CU count, HBM bytes. */
+vzStatus vzDeviceInfo(vzContext ctx, uint32_t* num_cus, uint64_t* hbm_bytes, char* name,
+                      size_t name_len);
+
+/* ---- scene and acceleration structure ------------------------------ */
+/* Copies n spheres/materials from host memory, uploads them and builds the
+ * LBVH on the GPU (replaces Renderer.h:160-255 and 452-520). */
+vzStatus vzSceneCreate(vzContext ctx, const vzSphere* spheres, const vzMaterial* materials,
+                       uint32_t n, vzStream stream, vzScene* out);
+vzStatus vzSceneDestroy(vzScene scene);
+/* Rebuild the LBVH from the current spheres. */
+vzStatus vzAccelBuild(vzScene scene, vzStream stream);
+/* Replace sphere geometry (same count, same materials) and refit the
+ * existing topology: boxes and node quantisation are recomputed bottom-up. */
+vzStatus vzSceneUpdateSpheres(vzScene scene, const vzSphere* spheres, vzStream stream);
+vzStatus vzAccelRefit(vzScene scene, vzStream stream);
+vzStatus vzAccelInfoGet(vzScene scene, vzAccelInfo* out);
+/* Copies the BVH to host memory for inspection (synchronises the stream).
+ * nodes: num_nodes * 32 bytes; leaf_prims, morton: num_prims words each.
+ * Any pointer may be NULL. */
+vzStatus vzAccelDownload(vzScene scene, void* nodes, uint32_t* leaf_prims, uint32_t* morton,
+                         vzStream stream);
+
+/* ---- tracing and rendering ----------------------------------------- */
+/* Closest hit for n rays (device arrays).  Equivalent of one optixTrace per
+ * ray with the given [tmin, tmax]; ties resolve to the smaller t, then the
+ * smaller primitive id. */
+vzStatus vzTraceClosest(vzScene scene, const vzRay* rays, uint32_t n, vzHit* hits,
+                        vzStream stream, vzStats* stats);
+/* One frame: the OptiX launch of Renderer::Draw (Renderer.h:75) plus the
+ * accumulation of __raygen__rg (RayTracer.cu:163-217).  Synchronises the
+ * stream when stats != NULL, otherwise returns after enqueueing. */
+vzStatus vzLaunch(vzScene scene, const vzLaunchParams* params, vzStream stream, vzStats* stats);
+
+/* ---- batched form --------------------------------------------------- */
+typedef struct vzBatchItem {
+    vzScene scene;
+    vzLaunchParams params;
+} vzBatchItem;
+
+typedef struct vzTuning {
+    uint32_t queue_capacity;  /* rays in flight (0 = default) */
+    uint32_t tile_w, tile_h;  /* camera-ray tile: tile_w*tile_h pixels per block of items */
+    uint32_t iters_per_graph; /* wavefront iterations captured per hipGraph (even) */
+    uint32_t stack_lds;       /* reserved */
+} vzTuning;
+
+/* Renders every item in one persistent wavefront captured as a hipGraph;
+ * items share the ray/shade/miss queues.  Synchronises the stream. */
+vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n,
+                       const vzTuning* tuning, vzStream stream, vzStats* stats);
+/* Times a short representative batch for each candidate tuning and returns
+ * the fastest; caches the choice per context. */
+vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStream stream,
+                    vzTuning* out);
+vzStatus vzDefaultTuning(vzContext ctx, vzTuning* out);
+
+/* ---- serving form --------------------------------------------------- */
+typedef struct vzServeConfig {
+    uint64_t hbm_budget_bytes;   /* admission budget for request buffers + scenes */
+    uint32_t max_active;         /* max requests in the device table (<= 1024) */
+    vzTuning tuning;
+} vzServeConfig;
+
+typedef struct vzRequestResult {
+    uint64_t request_id;
+    double submit_ms, admit_ms, finish_ms;  /* host clock, ms since server start */
+    uint64_t segments;
+    uint32_t status;                        /* VZ_SUCCESS or an error */
+} vzRequestResult;
+
+vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out);
+/* Enqueue a request; the accum/image buffers in params must stay valid until
+ * the request completes.  Returns immediately with its id. */
+vzStatus vzServerSubmit(vzServer srv, vzScene scene, const vzLaunchParams* params,
+                        uint64_t* request_id);
+/* Result of a finished request (VZ_ERROR_NOT_READY if still in flight). */
+vzStatus vzServerPoll(vzServer srv, uint64_t request_id, vzRequestResult* out);
+/* Block until every submitted request finished. */
+vzStatus vzServerDrain(vzServer srv);
+vzStatus vzServerDestroy(vzServer srv);
+/* Bytes of device memory a request of these params needs (admission). */
+uint64_t vzRequestFootprint(const vzLaunchParams* params);
+
+/* ---- host helpers (scene generators and camera, Scene.h / Camera.cpp) -- */
+/* Book final scene (Scene.h:13-80) from std::mt19937(seed), draws in
+ * x -> y -> z order.  Writes at most cap spheres; *n_out = count. */
+vzStatus vzGenerateBookScene(uint32_t seed, vzSphere* spheres, vzMaterial* materials,
+                             uint32_t cap, uint32_t* n_out);
+/* Ground sphere plus n_small random spheres in [-100,100]x[0,20]x[-100,100],
+ * radii log-uniform in [0.01, 0.5].  Writes n_small + 1 spheres. */
+vzStatus vzGenerateBoxScene(uint32_t n_small, uint32_t seed, vzSphere* spheres,
+                            vzMaterial* materials);
+/* Camera frame (Camera.cpp:24-37) into params->origin/u/v/w/lens_radius. */
+vzStatus vzCameraLookAt(const float lookfrom[3], const float lookat[3], float vfov_deg,
+                        float aspect, float aperture, float focus_dist, vzLaunchParams* params);
+
+/* ---- errors ----------------------------------------------------------- */
+const char* vzStatusString(vzStatus s);
+/* Message of the last failing call on this thread. */
+const char* vzGetLastError(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

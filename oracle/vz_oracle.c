@@ -1,0 +1,1017 @@
+/*
+ * vz_oracle.c -- CPU oracle for the sphere path tracer.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never a
+ * fallback for it.  See vz_oracle.h for what each function restates and what
+ * pins it.  Build: oracle/Makefile (gcc -O3 -ffp-contract=off).
+ */
+#define _GNU_SOURCE
+#include "vz_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+#define VZO_MAX_STACK 128
+
+struct vzo_bvh {
+    uint32_t n;           /* primitives */
+    vzo_node* nodes;      /* n-1 internal nodes, root = 0 */
+    uint32_t* leaf_prim;  /* n */
+    uint32_t* morton;     /* n, sorted */
+    float* boxes;         /* (n-1)*6 internal-node boxes lo xyz, hi xyz */
+    uint32_t depth;
+};
+
+/* ---------------------------------------------------------------- RNG --- */
+/* random.cuh:31-46: TEA with 4 rounds; returns v0 */
+uint32_t vzo_tea4(uint32_t v0, uint32_t v1)
+{
+    uint32_t s0 = 0;
+    for (int n = 0; n < 4; n++) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+
+/* random.cuh:48-55: Numerical Recipes LCG, low 24 bits returned */
+uint32_t vzo_lcg(uint32_t* state)
+{
+    *state = 1664525u * *state + 1013904223u;
+    return *state & 0x00FFFFFFu;
+}
+
+/* random.cuh:63-67: 24-bit integer / 2^24, exact in float */
+float vzo_rnd(uint32_t* state) { return (float)vzo_lcg(state) / (float)0x01000000; }
+
+/* ------------------------------------------------------------ mt19937 --- */
+/* Matsumoto & Nishimura 1998, the parameters std::mt19937 fixes. */
+typedef struct {
+    uint32_t mt[624];
+    int idx;
+} vzo_mt;
+
+static void mt_seed(vzo_mt* g, uint32_t seed)
+{
+    g->mt[0] = seed;
+    for (int i = 1; i < 624; i++) g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (uint32_t)i;
+    g->idx = 624;
+}
+
+static uint32_t mt_next(vzo_mt* g)
+{
+    if (g->idx >= 624) {
+        for (int i = 0; i < 624; i++) {
+            uint32_t y = (g->mt[i] & 0x80000000u) | (g->mt[(i + 1) % 624] & 0x7fffffffu);
+            g->mt[i] = g->mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        g->idx = 0;
+    }
+    uint32_t y = g->mt[g->idx++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+/* The build's uniform float: the top 24 bits of one output, exact in [0,1). */
+static float mt_uniform(vzo_mt* g) { return (float)(mt_next(g) >> 8) * (1.0f / 16777216.0f); }
+
+uint32_t vzo_mt19937_nth(uint32_t seed, uint32_t n)
+{
+    vzo_mt g;
+    mt_seed(&g, seed);
+    uint32_t v = 0;
+    for (uint32_t i = 0; i < n; i++) v = mt_next(&g);
+    return v;
+}
+
+/* ------------------------------------------------------------- scenes --- */
+static void set_mat(vzo_material* m, uint32_t type, float r, float g, float b, float fuzz, float ir)
+{
+    m->type = type;
+    m->albedo[0] = r;
+    m->albedo[1] = g;
+    m->albedo[2] = b;
+    m->fuzz = fuzz;
+    m->ir = ir;
+}
+
+static void set_sph(vzo_sphere* s, float x, float y, float z, float r)
+{
+    s->cx = x;
+    s->cy = y;
+    s->cz = z;
+    s->r = r;
+}
+
+/* Draw the material of one random sphere (Scene.h:31-57), x->y->z order. */
+static void draw_material(vzo_mt* g, float choose, vzo_material* m)
+{
+    if ((double)choose < 0.8) {
+        float a0 = mt_uniform(g), a1 = mt_uniform(g), a2 = mt_uniform(g);
+        float b0 = mt_uniform(g), b1 = mt_uniform(g), b2 = mt_uniform(g);
+        set_mat(m, VZO_LAMBERTIAN, a0 * b0, a1 * b1, a2 * b2, 0.0f, 0.0f);
+    } else if ((double)choose < 0.95) {
+        float r = 0.5f + (1.0f - 0.5f) * mt_uniform(g);
+        float gg = 0.5f + (1.0f - 0.5f) * mt_uniform(g);
+        float b = 0.5f + (1.0f - 0.5f) * mt_uniform(g);
+        float fuzz = 0.0f + (0.5f - 0.0f) * mt_uniform(g);
+        set_mat(m, VZO_METAL, r, gg, b, fuzz, 0.0f);
+    } else {
+        set_mat(m, VZO_DIELECTRIC, 0.0f, 0.0f, 0.0f, 0.0f, 1.5f);
+    }
+}
+
+/* Scene.h:13-80.  choose, then centre x, then centre z are drawn for every
+ * grid cell; material draws only for kept spheres. */
+uint32_t vzo_book_scene(uint32_t seed, vzo_sphere* s, vzo_material* m, uint32_t cap)
+{
+    vzo_mt g;
+    mt_seed(&g, seed);
+    uint32_t n = 0;
+#define PUSH(X, Y, Z, R)                         \
+    do {                                         \
+        if (n < cap) set_sph(&s[n], X, Y, Z, R); \
+    } while (0)
+    PUSH(0.0f, -1000.0f, 0.0f, 1000.0f);
+    if (n < cap) set_mat(&m[n], VZO_LAMBERTIAN, 0.5f, 0.5f, 0.5f, 0.0f, 0.0f);
+    n++;
+    for (int a = -11; a < 11; a++) {
+        for (int b = -11; b < 11; b++) {
+            float choose = mt_uniform(&g);
+            float ux = mt_uniform(&g);
+            float uz = mt_uniform(&g);
+            float cx = (float)((double)a + 0.9 * (double)ux);
+            float cy = 0.2f;
+            float cz = (float)((double)b + 0.9 * (double)uz);
+            float dx = cx - 4.0f, dy = cy - 0.2f, dz = cz - 0.0f;
+            float len = sqrtf(dx * dx + dy * dy + dz * dz);
+            if ((double)len > 0.9) {
+                vzo_material mm;
+                draw_material(&g, choose, &mm);
+                PUSH(cx, cy, cz, 0.2f);
+                if (n < cap) m[n] = mm;
+                n++;
+            }
+        }
+    }
+    PUSH(0.0f, 1.0f, 0.0f, 1.0f);
+    if (n < cap) set_mat(&m[n], VZO_DIELECTRIC, 0.0f, 0.0f, 0.0f, 0.0f, 1.5f);
+    n++;
+    PUSH(-4.0f, 1.0f, 0.0f, 1.0f);
+    if (n < cap) set_mat(&m[n], VZO_LAMBERTIAN, 0.4f, 0.2f, 0.1f, 0.0f, 0.0f);
+    n++;
+    PUSH(4.0f, 1.0f, 0.0f, 1.0f);
+    if (n < cap) set_mat(&m[n], VZO_METAL, 0.7f, 0.6f, 0.5f, 0.0f, 0.0f);
+    n++;
+#undef PUSH
+    return n;
+}
+
+/* Random-box scene: ground sphere, then n_small spheres with centres uniform
+ * in [-100,100]x[0,20]x[-100,100] and radii exp(U[ln 0.01, ln 0.5]).  Per
+ * sphere the draws are x, y, z, radius, choose, then the material draws. */
+void vzo_box_scene(uint32_t n_small, uint32_t seed, vzo_sphere* s, vzo_material* m)
+{
+    const float ln_lo = -0x1.26bb1cp+2f; /* logf(0.01f) */
+    const float ln_hi = -0x1.62e430p-1f; /* logf(0.5f)  */
+    vzo_mt g;
+    mt_seed(&g, seed);
+    set_sph(&s[0], 0.0f, -1000.0f, 0.0f, 1000.0f);
+    set_mat(&m[0], VZO_LAMBERTIAN, 0.5f, 0.5f, 0.5f, 0.0f, 0.0f);
+    for (uint32_t i = 1; i <= n_small; i++) {
+        float x = -100.0f + 200.0f * mt_uniform(&g);
+        float y = 0.0f + 20.0f * mt_uniform(&g);
+        float z = -100.0f + 200.0f * mt_uniform(&g);
+        float r = expf(ln_lo + (ln_hi - ln_lo) * mt_uniform(&g));
+        float choose = mt_uniform(&g);
+        set_sph(&s[i], x, y, z, r);
+        draw_material(&g, choose, &m[i]);
+    }
+}
+
+/* ------------------------------------------------------------- camera --- */
+static void n3(float v[3])
+{
+    float inv = 1.0f / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] *= inv;
+    v[1] *= inv;
+    v[2] *= inv;
+}
+
+static void cross3(const float a[3], const float b[3], float out[3])
+{
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* Camera.cpp:24-37, camera.h:20-24: W = forward*focus, U = norm(W x up),
+ * V = norm(U x W), then U *= focus*aspect*2h, V *= focus*2h. */
+void vzo_camera_lookat(const float from[3], const float at[3], float vfov_deg, float aspect,
+                       float aperture, float focus, vzo_camera* out)
+{
+    float f[3] = {at[0] - from[0], at[1] - from[1], at[2] - from[2]};
+    n3(f);
+    float W[3] = {f[0] * focus, f[1] * focus, f[2] * focus};
+    const float up[3] = {0.0f, 1.0f, 0.0f};
+    float U[3], V[3];
+    cross3(W, up, U);
+    n3(U);
+    cross3(U, W, V);
+    n3(V);
+    float theta = vfov_deg * 0.01745329251994329577f;
+    float h = tanf(theta * 0.5f);
+    float vh = 2.0f * h;
+    float vw = aspect * vh;
+    float su = focus * vw, sv = focus * vh;
+    for (int k = 0; k < 3; k++) {
+        out->origin[k] = from[k];
+        out->u[k] = U[k] * su;
+        out->v[k] = V[k] * sv;
+        out->w[k] = W[k];
+    }
+    out->lens_radius = aperture * 0.5f;
+}
+
+/* ------------------------------------------------------ node decoding --- */
+/* Per-axis scale 2^(e-127), e = biased exponent byte k of meta. */
+static inline float vzo_scale_of(uint32_t meta, int k)
+{
+    uint32_t e = (meta >> (8 * k)) & 0xFFu;
+    union {
+        uint32_t u;
+        float f;
+    } cv;
+    cv.u = e << 23;
+    return cv.f;
+}
+
+/* -------------------------------------------- real-generic instances --- */
+#define REAL float
+#define SUF _f32
+#define SQRT sqrtf
+#define FMA fmaf
+#define FMIN fminf
+#define FMAX fmaxf
+#define FABS fabsf
+#define COPYSIGN copysignf
+#define ROBUST (1.0f + 0x1p-20f)
+#include "vz_oracle_real.inc"
+#undef REAL
+#undef SUF
+#undef SQRT
+#undef FMA
+#undef FMIN
+#undef FMAX
+#undef FABS
+#undef COPYSIGN
+#undef ROBUST
+
+#define REAL double
+#define SUF _f64
+#define SQRT sqrt
+#define FMA fma
+#define FMIN fmin
+#define FMAX fmax
+#define FABS fabs
+#define COPYSIGN copysign
+#define ROBUST (1.0 + 0x1p-48)
+#include "vz_oracle_real.inc"
+#undef REAL
+#undef SUF
+#undef SQRT
+#undef FMA
+#undef FMIN
+#undef FMAX
+#undef FABS
+#undef COPYSIGN
+#undef ROBUST
+
+int vzo_hit_sphere_f32(const float o[3], const float d[3], float tmin, float tmax,
+                       const vzo_sphere* s, float* t)
+{
+    return hit_sphere_f32(mk_f32(o[0], o[1], o[2]), mk_f32(d[0], d[1], d[2]), tmin, tmax, s->cx,
+                          s->cy, s->cz, s->r, t);
+}
+
+int vzo_hit_sphere_f64(const double o[3], const double d[3], double tmin, double tmax,
+                       const vzo_sphere* s, double* t)
+{
+    return hit_sphere_f64(mk_f64(o[0], o[1], o[2]), mk_f64(d[0], d[1], d[2]), tmin, tmax, s->cx,
+                          s->cy, s->cz, s->r, t);
+}
+
+int vzo_scatter_f32(const vzo_material* m, const float d[3], const float n_out[3], int front,
+                    uint32_t* seed, float new_dir[3])
+{
+    v3_f32 nd = mk_f32(0.0f, 0.0f, 0.0f), att = nd;
+    int ok = scatter_f32(m, mk_f32(d[0], d[1], d[2]), mk_f32(n_out[0], n_out[1], n_out[2]), front,
+                         seed, &nd, &att);
+    new_dir[0] = nd.x;
+    new_dir[1] = nd.y;
+    new_dir[2] = nd.z;
+    return ok;
+}
+
+void vzo_sky_f32(const float d[3], float rgb[3])
+{
+    v3_f32 c = sky_f32(mk_f32(d[0], d[1], d[2]));
+    rgb[0] = c.x;
+    rgb[1] = c.y;
+    rgb[2] = c.z;
+}
+
+/* --------------------------------------------------------------- LBVH --- */
+static uint32_t expand10(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+/* 10 bits per axis of the centre relative to the centroid bounds:
+ * q = (int)((c - lo) * (1024 / ext)) clamped to [0, 1023]; x in the high
+ * bit of each triple. */
+static uint32_t morton_of(const vzo_sphere* s, const float lo[3], const float sc[3])
+{
+    float c[3] = {s->cx, s->cy, s->cz};
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        float f = (c[k] - lo[k]) * sc[k];
+        int qi = (int)f;
+        if (qi < 0) qi = 0;
+        if (qi > 1023) qi = 1023;
+        q[k] = (uint32_t)qi;
+    }
+    return (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+}
+
+static inline int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+
+/* Karras 2012 delta: common prefix length of keys i and j, with the index
+ * appended to break ties; -1 outside [0, n). */
+static inline int kdelta(const uint32_t* k, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a == b) return 32 + clz32((uint32_t)i ^ (uint32_t)j);
+    return clz32(a ^ b);
+}
+
+/* Stable LSD radix sort of (key, value), 8-bit digits. */
+static void radix_sort(uint32_t* key, uint32_t* val, uint32_t n)
+{
+    uint32_t* k2 = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* v2 = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    for (int pass = 0; pass < 4; pass++) {
+        uint32_t cnt[257];
+        memset(cnt, 0, sizeof(cnt));
+        int sh = pass * 8;
+        for (uint32_t i = 0; i < n; i++) cnt[((key[i] >> sh) & 0xFF) + 1]++;
+        for (int d = 0; d < 256; d++) cnt[d + 1] += cnt[d];
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t d = (key[i] >> sh) & 0xFF;
+            uint32_t pos = cnt[d]++;
+            k2[pos] = key[i];
+            v2[pos] = val[i];
+        }
+        memcpy(key, k2, sizeof(uint32_t) * n);
+        memcpy(val, v2, sizeof(uint32_t) * n);
+    }
+    free(k2);
+    free(v2);
+}
+
+static float f_as(uint32_t u)
+{
+    union {
+        uint32_t u;
+        float f;
+    } cv;
+    cv.u = u;
+    return cv.f;
+}
+static uint32_t u_as(float f)
+{
+    union {
+        uint32_t u;
+        float f;
+    } cv;
+    cv.f = f;
+    return cv.u;
+}
+
+/* Biased exponent e (>= 1) of the smallest power of two s with
+ * fl(lo + s*255) >= hi. */
+static uint32_t pick_exp(float lo, float hi)
+{
+    float ext = hi - lo;
+    uint32_t e = 1;
+    if (ext > 0.0f) {
+        float m = ext / 255.0f;
+        uint32_t b = u_as(m);
+        uint32_t ex = b >> 23, mant = b & 0x7FFFFFu;
+        e = (ex == 0) ? 1u : ex + (mant != 0 ? 1u : 0u);
+        if (e < 1) e = 1;
+    }
+    while (e < 254 && fmaf(f_as(e << 23), 255.0f, lo) < hi) e++;
+    return e;
+}
+
+static uint32_t q_lo(float v, float lo, float s)
+{
+    float qf = floorf((v - lo) / s);
+    int q = (int)qf;
+    if (q < 0) q = 0;
+    if (q > 255) q = 255;
+    while (q > 0 && fmaf(s, (float)q, lo) > v) q--;
+    return (uint32_t)q;
+}
+
+static uint32_t q_hi(float v, float lo, float s)
+{
+    float qf = ceilf((v - lo) / s);
+    int q = (int)qf;
+    if (q < 0) q = 0;
+    if (q > 255) q = 255;
+    while (q < 255 && fmaf(s, (float)q, lo) < v) q++;
+    return (uint32_t)q;
+}
+
+/* Pack internal node i's two children (boxes cb[0], cb[1]) relative to its
+ * own box nb into the 32-byte node. */
+static void pack_node(vzo_node* N, const float nb[6], const float cb0[6], const float cb1[6],
+                      uint32_t child, uint32_t flags)
+{
+    uint32_t meta = 0;
+    for (int k = 0; k < 3; k++) {
+        float lo = nb[k], hi = nb[3 + k];
+        uint32_t e = pick_exp(lo, hi);
+        float s = f_as(e << 23);
+        meta |= e << (8 * k);
+        uint32_t l0 = q_lo(cb0[k], lo, s), h0 = q_hi(cb0[3 + k], lo, s);
+        uint32_t l1 = q_lo(cb1[k], lo, s), h1 = q_hi(cb1[3 + k], lo, s);
+        N->o[k] = lo;
+        N->q[k] = l0 | (h0 << 8) | (l1 << 16) | (h1 << 24);
+    }
+    N->meta = meta | (flags << 24);
+    N->child = child;
+}
+
+static void leaf_box(const vzo_sphere* s, float b[6])
+{
+    float r = fabsf(s->r);
+    b[0] = s->cx - r;
+    b[1] = s->cy - r;
+    b[2] = s->cz - r;
+    b[3] = s->cx + r;
+    b[4] = s->cy + r;
+    b[5] = s->cz + r;
+}
+
+vzo_bvh* vzo_bvh_build(const vzo_sphere* s, uint32_t n)
+{
+    if (n == 0) return NULL;
+    vzo_bvh* b = (vzo_bvh*)calloc(1, sizeof(vzo_bvh));
+    b->n = n;
+    b->leaf_prim = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    b->morton = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t ni = n > 1 ? n - 1 : 0;
+    b->nodes = (vzo_node*)calloc(ni ? ni : 1, sizeof(vzo_node));
+    b->boxes = (float*)calloc((ni ? ni : 1) * 6, sizeof(float));
+    /* centroid bounds */
+    float lo[3] = {s[0].cx, s[0].cy, s[0].cz}, hi[3] = {s[0].cx, s[0].cy, s[0].cz};
+    for (uint32_t i = 1; i < n; i++) {
+        float c[3] = {s[i].cx, s[i].cy, s[i].cz};
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], c[k]);
+            hi[k] = fmaxf(hi[k], c[k]);
+        }
+    }
+    float sc[3];
+    for (int k = 0; k < 3; k++) {
+        float ext = hi[k] - lo[k];
+        sc[k] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        b->morton[i] = morton_of(&s[i], lo, sc);
+        b->leaf_prim[i] = i;
+    }
+    radix_sort(b->morton, b->leaf_prim, n);
+    b->depth = 1;
+    if (n == 1) return b;
+
+    /* Karras 2012, algorithm 4 */
+    uint32_t* parent_int = (uint32_t*)malloc(sizeof(uint32_t) * ni);
+    uint32_t* parent_leaf = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* gam = (uint32_t*)malloc(sizeof(uint32_t) * ni);
+    uint32_t* flg = (uint32_t*)malloc(sizeof(uint32_t) * ni);
+    const uint32_t* K = b->morton;
+    int nn = (int)n;
+    parent_int[0] = 0xFFFFFFFFu;
+    for (int i = 0; i < nn - 1; i++) {
+        int d = (kdelta(K, nn, i, i + 1) - kdelta(K, nn, i, i - 1)) > 0 ? 1 : -1;
+        int dmin = kdelta(K, nn, i, i - d);
+        int lmax = 2;
+        while (kdelta(K, nn, i, i + lmax * d) > dmin) lmax *= 2;
+        int l = 0;
+        for (int t = lmax / 2; t >= 1; t /= 2)
+            if (kdelta(K, nn, i, i + (l + t) * d) > dmin) l += t;
+        int j = i + l * d;
+        int dnode = kdelta(K, nn, i, j);
+        int sx = 0;
+        int div = 2;
+        int t = (l + div - 1) / div;
+        for (;;) {
+            if (kdelta(K, nn, i, i + (sx + t) * d) > dnode) sx += t;
+            if (t == 1) break;
+            div *= 2;
+            t = (l + div - 1) / div;
+        }
+        int g = i + sx * d + (d < 0 ? d : 0);
+        int mn = i < j ? i : j, mx = i < j ? j : i;
+        uint32_t f = 0;
+        if (mn == g) {
+            f |= 1u;
+            parent_leaf[g] = (uint32_t)i;
+        } else {
+            parent_int[g] = (uint32_t)i;
+        }
+        if (mx == g + 1) {
+            f |= 2u;
+            parent_leaf[g + 1] = (uint32_t)i;
+        } else {
+            parent_int[g + 1] = (uint32_t)i;
+        }
+        gam[i] = (uint32_t)g;
+        flg[i] = f;
+    }
+    /* bottom-up boxes: a node is finished when both children are */
+    uint8_t* seen = (uint8_t*)calloc(ni, 1);
+    for (uint32_t leaf = 0; leaf < n; leaf++) {
+        uint32_t p = parent_leaf[leaf];
+        for (;;) {
+            if (++seen[p] < 2) break;
+            float cb[2][6];
+            for (int c = 0; c < 2; c++) {
+                uint32_t ch = gam[p] + (uint32_t)c;
+                if ((flg[p] >> c) & 1u)
+                    leaf_box(&s[b->leaf_prim[ch]], cb[c]);
+                else
+                    memcpy(cb[c], &b->boxes[ch * 6], sizeof(float) * 6);
+            }
+            float* nb = &b->boxes[p * 6];
+            for (int k = 0; k < 3; k++) {
+                nb[k] = fminf(cb[0][k], cb[1][k]);
+                nb[3 + k] = fmaxf(cb[0][3 + k], cb[1][3 + k]);
+            }
+            pack_node(&b->nodes[p], nb, cb[0], cb[1], gam[p], flg[p]);
+            if (p == 0) break;
+            p = parent_int[p];
+        }
+    }
+    /* depth */
+    {
+        uint32_t stk[VZO_MAX_STACK * 2], dep[VZO_MAX_STACK * 2];
+        int sp = 0;
+        stk[sp] = 0;
+        dep[sp] = 1;
+        sp++;
+        uint32_t maxd = 1;
+        while (sp > 0) {
+            sp--;
+            uint32_t nd = stk[sp], dd = dep[sp];
+            if (dd + 1 > maxd) maxd = dd + 1;
+            for (int c = 0; c < 2; c++)
+                if (!((flg[nd] >> c) & 1u) && sp < VZO_MAX_STACK * 2) {
+                    stk[sp] = gam[nd] + (uint32_t)c;
+                    dep[sp] = dd + 1;
+                    sp++;
+                }
+        }
+        b->depth = maxd;
+    }
+    free(seen);
+    free(parent_int);
+    free(parent_leaf);
+    free(gam);
+    free(flg);
+    return b;
+}
+
+void vzo_bvh_free(vzo_bvh* b)
+{
+    if (!b) return;
+    free(b->nodes);
+    free(b->leaf_prim);
+    free(b->morton);
+    free(b->boxes);
+    free(b);
+}
+
+uint32_t vzo_bvh_num_prims(const vzo_bvh* b) { return b->n; }
+uint32_t vzo_bvh_num_nodes(const vzo_bvh* b) { return b->n > 1 ? b->n - 1 : 0; }
+const vzo_node* vzo_bvh_nodes(const vzo_bvh* b) { return b->nodes; }
+const uint32_t* vzo_bvh_leaf_prims(const vzo_bvh* b) { return b->leaf_prim; }
+const uint32_t* vzo_bvh_morton(const vzo_bvh* b) { return b->morton; }
+const float* vzo_bvh_node_boxes(const vzo_bvh* b) { return b->boxes; }
+uint32_t vzo_bvh_depth(const vzo_bvh* b) { return b->depth; }
+
+/* Structural invariants: each leaf and each non-root internal node is
+ * referenced exactly once; parent boxes contain child boxes; dequantised
+ * child boxes contain the exact child boxes; Morton order is sorted and
+ * stable; each split is where the node's common prefix ends. */
+int vzo_bvh_validate(const vzo_bvh* b, const vzo_sphere* s)
+{
+    uint32_t n = b->n;
+    for (uint32_t i = 1; i < n; i++) {
+        if (b->morton[i - 1] > b->morton[i]) return 1;
+        if (b->morton[i - 1] == b->morton[i] && b->leaf_prim[i - 1] > b->leaf_prim[i]) return 2;
+    }
+    if (n == 1) return 0;
+    uint32_t ni = n - 1;
+    uint8_t* ref_leaf = (uint8_t*)calloc(n, 1);
+    uint8_t* ref_int = (uint8_t*)calloc(ni, 1);
+    int rc = 0;
+    for (uint32_t i = 0; i < ni && !rc; i++) {
+        const vzo_node* N = &b->nodes[i];
+        uint32_t flags = N->meta >> 24;
+        for (int c = 0; c < 2; c++) {
+            uint32_t ch = N->child + (uint32_t)c;
+            float cb[6];
+            if ((flags >> c) & 1u) {
+                if (ch >= n) { rc = 3; break; }
+                ref_leaf[ch]++;
+                leaf_box(&s[b->leaf_prim[ch]], cb);
+            } else {
+                if (ch >= ni || ch == 0) { rc = 4; break; }
+                ref_int[ch]++;
+                memcpy(cb, &b->boxes[ch * 6], sizeof(cb));
+            }
+            for (int k = 0; k < 3; k++) {
+                float sc = vzo_scale_of(N->meta, k);
+                float dl = fmaf(sc, (float)((N->q[k] >> (16 * c)) & 0xFF), N->o[k]);
+                float dh = fmaf(sc, (float)((N->q[k] >> (16 * c + 8)) & 0xFF), N->o[k]);
+                if (!(dl <= cb[k] && dh >= cb[3 + k])) rc = 5;
+                if (!(b->boxes[i * 6 + k] <= cb[k] && b->boxes[i * 6 + 3 + k] >= cb[3 + k])) rc = 6;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < n && !rc; i++)
+        if (ref_leaf[i] != 1) rc = 7;
+    for (uint32_t i = 1; i < ni && !rc; i++)
+        if (ref_int[i] != 1) rc = 8;
+    /* split property via leaf ranges */
+    if (!rc) {
+        uint32_t stk[256], fst[256], lst[256];
+        int sp = 0;
+        stk[0] = 0;
+        fst[0] = 0;
+        lst[0] = n - 1;
+        sp = 1;
+        int nn = (int)n;
+        while (sp > 0 && !rc) {
+            sp--;
+            uint32_t nd = stk[sp], f = fst[sp], l = lst[sp];
+            const vzo_node* N = &b->nodes[nd];
+            uint32_t g = N->child;
+            if (!(g >= f && g < l)) { rc = 9; break; }
+            int dn = kdelta(b->morton, nn, (int)f, (int)l);
+            if (kdelta(b->morton, nn, (int)g, (int)g + 1) != dn) { rc = 10; break; }
+            uint32_t flags = N->meta >> 24;
+            if (((flags & 1u) != 0) != (g == f)) { rc = 11; break; }
+            if (((flags & 2u) != 0) != (g + 1 == l)) { rc = 12; break; }
+            if (!(flags & 1u)) {
+                if (sp >= 255) { rc = 13; break; }
+                stk[sp] = g; fst[sp] = f; lst[sp] = g; sp++;
+            }
+            if (!(flags & 2u)) {
+                if (sp >= 255) { rc = 13; break; }
+                stk[sp] = g + 1; fst[sp] = g + 1; lst[sp] = l; sp++;
+            }
+        }
+    }
+    free(ref_leaf);
+    free(ref_int);
+    return rc;
+}
+
+void vzo_trace(const vzo_bvh* b, const vzo_sphere* s, const vzo_ray* rays, uint32_t n,
+               vzo_hit* hits, uint64_t* node_visits, uint64_t* sphere_tests)
+{
+    uint64_t nv = 0, ns = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const vzo_ray* r = &rays[i];
+        float t;
+        uint32_t p;
+        closest_f32(b, s, mk_f32(r->ox, r->oy, r->oz), mk_f32(r->dx, r->dy, r->dz), r->tmin, r->tmax,
+                    &t, &p, &nv, &ns);
+        hits[i].t = (p == VZO_MISS) ? r->tmax : t;
+        hits[i].prim = p;
+    }
+    if (node_visits) *node_visits = nv;
+    if (sphere_tests) *sphere_tests = ns;
+}
+
+void vzo_trace_brute(const vzo_sphere* s, uint32_t nsph, const vzo_ray* rays, uint32_t n,
+                     vzo_hit* hits)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        const vzo_ray* r = &rays[i];
+        float best = r->tmax;
+        uint32_t bp = VZO_MISS;
+        v3_f32 o = mk_f32(r->ox, r->oy, r->oz), d = mk_f32(r->dx, r->dy, r->dz);
+        for (uint32_t j = 0; j < nsph; j++) {
+            float t;
+            if (hit_sphere_f32(o, d, r->tmin, best, s[j].cx, s[j].cy, s[j].cz, s[j].r, &t)) {
+                if (t < best || j < bp) {
+                    best = t;
+                    bp = j;
+                }
+            }
+        }
+        hits[i].t = best;
+        hits[i].prim = bp;
+    }
+}
+
+void vzo_trace_brute_f64(const vzo_sphere* s, uint32_t nsph, const vzo_ray* rays, uint32_t n,
+                         double* t_out, uint32_t* prim_out)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        const vzo_ray* r = &rays[i];
+        double best = r->tmax;
+        uint32_t bp = VZO_MISS;
+        v3_f64 o = mk_f64(r->ox, r->oy, r->oz), d = mk_f64(r->dx, r->dy, r->dz);
+        for (uint32_t j = 0; j < nsph; j++) {
+            double t;
+            if (hit_sphere_f64(o, d, r->tmin, best, s[j].cx, s[j].cy, s[j].cz, s[j].r, &t)) {
+                if (t < best || j < bp) {
+                    best = t;
+                    bp = j;
+                }
+            }
+        }
+        t_out[i] = best;
+        prim_out[i] = bp;
+    }
+}
+
+/* ------------------------------------------------------ sRGB encoding --- */
+/* RayTracer.cu:16-43: clamp, sRGB with exponent 1/2.4, x*256 truncated,
+ * clamped to 255. */
+uint8_t vzo_encode8(float c)
+{
+    c = fmaxf(0.0f, fminf(c, 1.0f));
+    float s = c < 0.0031308f ? 12.92f * c : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+    s = fmaxf(0.0f, fminf(s, 1.0f));
+    unsigned int q = (unsigned int)(s * 256.0f);
+    return (uint8_t)(q < 255u ? q : 255u);
+}
+
+/* ---------------------------------------------------------- rendering --- */
+typedef struct {
+    const vzo_bvh* b;
+    const vzo_sphere* s;
+    const vzo_material* m;
+    uint32_t nsph;
+    const vzo_render_params* p;
+    void* accum;
+    uint8_t* image;
+    int f64;
+    uint32_t ntiles_x, ntiles_y, nrows;
+    volatile uint32_t next;
+    vzo_stats st[256];
+} job_t;
+
+typedef struct {
+    job_t* j;
+    int id;
+} worker_t;
+
+/* Progressive blend (RayTracer.cu:206-216): avg = sum * (1/spp); for
+ * subframe > 1 the result is prev + (1/subframe)*(avg - prev), else avg. */
+static void finish_pixel(job_t* j, uint32_t idx, const float sum32[3], const double sum64[3],
+                         const double sumf64[3])
+{
+    const vzo_render_params* p = j->p;
+    float outc[3];
+    if (j->f64) {
+        double* acc = (double*)j->accum + 4 * (size_t)idx;
+        double avg[3];
+        for (int k = 0; k < 3; k++) avg[k] = sumf64[k] / (double)p->spp;
+        if (p->subframe > 1) {
+            double a = 1.0 / (double)p->subframe;
+            for (int k = 0; k < 3; k++) avg[k] = acc[k] + a * (avg[k] - acc[k]);
+        }
+        for (int k = 0; k < 3; k++) acc[k] = avg[k];
+        acc[3] = 1.0;
+        for (int k = 0; k < 3; k++) outc[k] = (float)avg[k];
+    } else if (p->accum_fp64) {
+        double avg[3];
+        for (int k = 0; k < 3; k++) avg[k] = sum64[k] / (double)p->spp;
+        double* acc = (double*)j->accum + 4 * (size_t)idx;
+        if (p->subframe > 1 && acc) {
+            double a = 1.0 / (double)p->subframe;
+            for (int k = 0; k < 3; k++) avg[k] = acc[k] + a * (avg[k] - acc[k]);
+        }
+        if (j->accum) {
+            for (int k = 0; k < 3; k++) acc[k] = avg[k];
+            acc[3] = 1.0;
+        }
+        for (int k = 0; k < 3; k++) outc[k] = (float)avg[k];
+    } else {
+        float inv = 1.0f / (float)p->spp;
+        float avg[3] = {sum32[0] * inv, sum32[1] * inv, sum32[2] * inv};
+        float* acc = j->accum ? (float*)j->accum + 4 * (size_t)idx : NULL;
+        if (p->subframe > 1 && acc) {
+            float a = 1.0f / (float)p->subframe;
+            for (int k = 0; k < 3; k++) avg[k] = acc[k] + a * (avg[k] - acc[k]);
+        }
+        if (acc) {
+            for (int k = 0; k < 3; k++) acc[k] = avg[k];
+            acc[3] = 1.0f;
+        }
+        for (int k = 0; k < 3; k++) outc[k] = avg[k];
+    }
+    if (j->image) {
+        uint8_t* px = j->image + 4 * (size_t)idx;
+        px[0] = vzo_encode8(outc[0]);
+        px[1] = vzo_encode8(outc[1]);
+        px[2] = vzo_encode8(outc[2]);
+        px[3] = 255;
+    }
+}
+
+static void* worker(void* arg)
+{
+    worker_t* w = (worker_t*)arg;
+    job_t* j = w->j;
+    const vzo_render_params* p = j->p;
+    vzo_stats* st = &j->st[w->id];
+    pctx_f32 c32 = {j->b, j->s, j->m, j->nsph, p->max_depth, NULL, 0, 0};
+    pctx_f64 c64 = {j->b, j->s, j->m, j->nsph, p->max_depth, NULL, 0, 0};
+    uint32_t ntiles = j->ntiles_x * j->ntiles_y;
+    for (;;) {
+        uint32_t t = __atomic_fetch_add(&j->next, 1u, __ATOMIC_RELAXED);
+        if (t >= ntiles) break;
+        uint32_t tx = t % j->ntiles_x, ty = t / j->ntiles_x;
+        for (uint32_t ry = ty * 16; ry < ty * 16 + 16 && ry < j->nrows; ry++) {
+            uint32_t y = p->row_start + ry * p->row_step;
+            for (uint32_t x = tx * 16; x < tx * 16 + 16 && x < p->width; x++) {
+                uint32_t idx = y * p->width + x;
+                float s32[3] = {0, 0, 0};
+                double s64[3] = {0, 0, 0}, sf64[3] = {0, 0, 0};
+                if (j->f64) {
+                    double d[3];
+                    pixel_f64(&c64, p, x, y, sf64, d, st);
+                } else {
+                    pixel_f32(&c32, p, x, y, s32, s64, st);
+                }
+                finish_pixel(j, idx, s32, s64, sf64);
+            }
+        }
+    }
+    return NULL;
+}
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void run_job(job_t* j, vzo_stats* out)
+{
+    const vzo_render_params* p = j->p;
+    uint32_t step = p->row_step ? p->row_step : 1;
+    j->nrows = p->row_start < p->height ? (p->height - p->row_start + step - 1) / step : 0;
+    j->ntiles_x = (p->width + 15) / 16;
+    j->ntiles_y = (j->nrows + 15) / 16;
+    j->next = 0;
+    int nt = (int)p->threads;
+    if (nt <= 0) nt = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (nt < 1) nt = 1;
+    if (nt > 256) nt = 256;
+    memset(j->st, 0, sizeof(j->st));
+    pthread_t th[256];
+    worker_t w[256];
+    double t0 = now_s();
+    for (int i = 0; i < nt; i++) {
+        w[i].j = j;
+        w[i].id = i;
+        if (i > 0) pthread_create(&th[i], NULL, worker, &w[i]);
+    }
+    worker(&w[0]);
+    for (int i = 1; i < nt; i++) pthread_join(th[i], NULL);
+    double t1 = now_s();
+    if (out) {
+        memset(out, 0, sizeof(*out));
+        for (int i = 0; i < nt; i++) {
+            out->segments += j->st[i].segments;
+            out->paths += j->st[i].paths;
+            out->hits += j->st[i].hits;
+            out->node_visits += j->st[i].node_visits;
+            out->sphere_tests += j->st[i].sphere_tests;
+        }
+        out->seconds = t1 - t0;
+        out->threads = (uint32_t)nt;
+    }
+}
+
+static int params_ok(const vzo_render_params* p)
+{
+    return p->width >= 2 && p->height >= 2 && p->spp >= 1 && p->max_depth >= 1;
+}
+
+void vzo_render(const vzo_bvh* b, const vzo_sphere* s, const vzo_material* m, uint32_t nsph,
+                const vzo_render_params* p, void* accum, uint8_t* image, vzo_stats* st)
+{
+    if (!params_ok(p)) return;
+    job_t* j = (job_t*)calloc(1, sizeof(job_t));
+    j->b = b;
+    j->s = s;
+    j->m = m;
+    j->nsph = nsph;
+    j->p = p;
+    j->accum = accum;
+    j->image = image;
+    j->f64 = 0;
+    run_job(j, st);
+    free(j);
+}
+
+void vzo_render_f64(const vzo_bvh* b, const vzo_sphere* s, const vzo_material* m, uint32_t nsph,
+                    const vzo_render_params* p, double* accum, vzo_stats* st)
+{
+    if (!params_ok(p) || !accum) return;
+    job_t* j = (job_t*)calloc(1, sizeof(job_t));
+    j->b = b;
+    j->s = s;
+    j->m = m;
+    j->nsph = nsph;
+    j->p = p;
+    j->accum = accum;
+    j->image = NULL;
+    j->f64 = 1;
+    run_job(j, st);
+    free(j);
+}
+
+void vzo_make_rayset(const vzo_bvh* b, const vzo_sphere* s, const vzo_material* m, uint32_t nsph,
+                     const vzo_render_params* p, uint32_t n_int, uint32_t n_iso, uint32_t iso_seed,
+                     vzo_ray* out)
+{
+    pctx_f32 c = {b, s, m, nsph, p->max_depth, out, 0, n_int};
+    vzo_stats st;
+    memset(&st, 0, sizeof(st));
+    for (uint32_t y = 0; y < p->height && c.rec_n < n_int; y++)
+        for (uint32_t x = 0; x < p->width && c.rec_n < n_int; x++) {
+            float s32[3];
+            double s64[3];
+            pixel_f32(&c, p, x, y, s32, s64, &st);
+        }
+    /* isotropic rays from points in the AABB of spheres 1..nsph-1 */
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t i = (nsph > 1 ? 1 : 0); i < nsph; i++) {
+        float bx[6];
+        leaf_box(&s[i], bx);
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], bx[k]);
+            hi[k] = fmaxf(hi[k], bx[3 + k]);
+        }
+    }
+    vzo_mt g;
+    mt_seed(&g, iso_seed);
+    for (uint32_t i = 0; i < n_iso; i++) {
+        vzo_ray* r = &out[n_int + i];
+        r->ox = lo[0] + (hi[0] - lo[0]) * mt_uniform(&g);
+        r->oy = lo[1] + (hi[1] - lo[1]) * mt_uniform(&g);
+        r->oz = lo[2] + (hi[2] - lo[2]) * mt_uniform(&g);
+        float d[3];
+        for (;;) {
+            d[0] = -1.0f + 2.0f * mt_uniform(&g);
+            d[1] = -1.0f + 2.0f * mt_uniform(&g);
+            d[2] = -1.0f + 2.0f * mt_uniform(&g);
+            float l2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (l2 < 1.0f && l2 > 1e-6f) break;
+        }
+        n3(d);
+        r->dx = d[0];
+        r->dy = d[1];
+        r->dz = d[2];
+        r->tmin = 0.001f;
+        r->tmax = 1e16f;
+    }
+}

@@ -1,0 +1,220 @@
+// vz_device.h -- device-side math of the path tracer (gfx950).
+//
+// Every expression is evaluated in the order written (the library is built
+// with -ffp-contract=off and IEEE-rounded divide/sqrt), so the GPU computes
+// the same float results as the CPU integrator that checks it.  Reference
+// semantics followed:
+//   RNG ............ Core/random.cuh:31-67
+//   vec maths ...... Core/vec_math.h:412-570
+//   samplers ....... Core/RayTracer.cu:87-149
+//   camera ray ..... Core/RayTracer.cu:151-177
+//   sphere test .... Core/RayTracer.cu:229-270 (compensated discriminant,
+//                    stable roots)
+//   materials ...... Core/RayTracer.cu:272-440
+//   miss ........... Core/RayTracer.cu:442-450
+//   make_color ..... Core/RayTracer.cu:16-47
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define VZ_DEV __device__ __forceinline__
+
+namespace vz {
+
+struct v3 {
+    float x, y, z;
+};
+
+VZ_DEV v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+VZ_DEV v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+VZ_DEV v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+VZ_DEV v3 mul(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+VZ_DEV v3 scl(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+VZ_DEV v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+VZ_DEV float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+VZ_DEV v3 normalize(v3 v) { float inv = 1.0f / sqrtf(dot(v, v)); return scl(v, inv); }
+VZ_DEV v3 reflect(v3 i, v3 n) { return sub(i, scl(scl(n, 2.0f), dot(n, i))); }
+VZ_DEV v3 refract(v3 uv, v3 n, float eta)
+{
+    float ct = fminf(dot(neg(uv), n), 1.0f);
+    v3 perp = scl(add(uv, scl(n, ct)), eta);
+    float k = -sqrtf(fabsf(1.0f - dot(perp, perp)));
+    return add(perp, scl(n, k));
+}
+
+// ---- RNG (random.cuh:31-67) -------------------------------------------
+VZ_DEV uint32_t tea4(uint32_t v0, uint32_t v1)
+{
+    uint32_t s0 = 0;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+VZ_DEV float rnd(uint32_t& s)
+{
+    s = 1664525u * s + 1013904223u;
+    return (float)(s & 0x00FFFFFFu) * (1.0f / 16777216.0f);  // exact: 24-bit int * 2^-24
+}
+VZ_DEV float rnd_m11(uint32_t& s) { return -1.0f + 2.0f * rnd(s); }
+
+VZ_DEV v3 in_unit_sphere(uint32_t& s)
+{
+    for (;;) {
+        float x = rnd_m11(s);
+        float y = rnd_m11(s);
+        float z = rnd_m11(s);
+        v3 p = mk(x, y, z);
+        if (dot(p, p) >= 1.0f) continue;
+        return p;
+    }
+}
+
+// |x| < 1e-8 in double  <=>  |x| <= 1e-8f in float (1e-8f rounds below 1e-8)
+VZ_DEV bool near_zero(v3 e)
+{
+    return (fabsf(e.x) <= 1e-8f) && (fabsf(e.y) <= 1e-8f) && (fabsf(e.z) <= 1e-8f);
+}
+
+// ---- sphere test ---------------------------------------------------------
+// Kahan's FMA difference of products for hb*hb - a*cc, stable root pair
+// q/a and cc/q, nearer root in [tmin, tmax] else the farther one.
+VZ_DEV bool hit_sphere(v3 o, v3 d, float tmin, float tmax, float4 sph, float& tout)
+{
+    v3 oc = sub(o, mk(sph.x, sph.y, sph.z));
+    float a = dot(d, d);
+    float hb = dot(oc, d);
+    float cc = dot(oc, oc) - sph.w * sph.w;
+    float w = a * cc;
+    float e = __builtin_fmaf(-a, cc, w);
+    float f = __builtin_fmaf(hb, hb, -w);
+    float disc = f + e;
+    if (disc < 0.0f) return false;
+    float sq = sqrtf(disc);
+    float q = -(hb + copysignf(sq, hb));
+    if (q == 0.0f) return false;
+    float r1 = q / a;
+    float r2 = cc / q;
+    float tn = fminf(r1, r2);
+    float tf = fmaxf(r1, r2);
+    float t = tn;
+    if (t < tmin || tmax < t) {
+        t = tf;
+        if (t < tmin || tmax < t) return false;
+    }
+    tout = t;
+    return true;
+}
+
+// ---- miss (RayTracer.cu:442-450) -----------------------------------------
+VZ_DEV v3 sky(v3 d)
+{
+    v3 u = normalize(d);
+    float t = 0.5f * (u.y + 1.0f);
+    return mk(1.0f + t * (0.5f - 1.0f), 1.0f + t * (0.7f - 1.0f), 1.0f + t * (1.0f - 1.0f));
+}
+
+// ---- materials (RayTracer.cu:272-440) --------------------------------------
+// mat = {albedo rgb, fuzz (metal) or ior (dielectric)}.  Returns false when
+// the path is absorbed.  att is the throughput multiplier.
+VZ_DEV bool scatter(uint32_t type, float4 mat, v3 d, v3 n, bool front, uint32_t& seed, v3& nd,
+                    v3& att)
+{
+    if (type == 0u) {
+        v3 us = normalize(in_unit_sphere(seed));
+        v3 sd = add(n, us);
+        if (near_zero(sd)) sd = n;
+        nd = sd;
+        att = mk(mat.x, mat.y, mat.z);
+        return true;
+    } else if (type == 1u) {
+        v3 refl = reflect(normalize(d), n);
+        v3 rs = in_unit_sphere(seed);
+        v3 sd = add(refl, scl(rs, mat.w));
+        if (dot(sd, n) > 0.0f) {
+            nd = sd;
+            att = mk(mat.x, mat.y, mat.z);
+            return true;
+        }
+        return false;
+    } else {
+        float ir = mat.w;
+        float ratio = front ? (1.0f / ir) : ir;
+        v3 u = normalize(d);
+        float ct = fminf(dot(neg(u), n), 1.0f);
+        float st = sqrtf(1.0f - ct * ct);
+        bool cannot = ratio * st > 1.0f;
+        bool refl = cannot;
+        if (!cannot) {
+            float r0 = (1.0f - ratio) / (1.0f + ratio);
+            r0 = r0 * r0;
+            float x = 1.0f - ct;
+            float x2 = x * x;
+            float x5 = (x2 * x2) * x;
+            float sch = r0 + (1.0f - r0) * x5;
+            refl = sch > rnd(seed);
+        }
+        nd = refl ? reflect(u, n) : refract(u, n, ratio);
+        att = mk(1.0f, 1.0f, 1.0f);
+        return true;
+    }
+}
+
+// ---- camera (RayTracer.cu:141-177) ---------------------------------------
+struct Cam {
+    v3 O, U, V, W, un, vn;
+    float lr;
+    uint32_t width, height;
+};
+
+// Draws jitter and the lens sample from seed (advancing it) and returns the
+// camera ray of pixel (x, y).
+VZ_DEV void camera_ray(const Cam& c, uint32_t x, uint32_t y, uint32_t& seed, v3& org, v3& dir)
+{
+    float jx = rnd(seed);
+    float jy = rnd(seed);
+    float u = 2.0f * ((float)x + jx) / (float)(c.width - 1) - 1.0f;
+    float v = 2.0f * ((float)y + jy) / (float)(c.height - 1) - 1.0f;
+    float dx, dy;
+    for (;;) {
+        dx = rnd_m11(seed);
+        dy = rnd_m11(seed);
+        if (dx * dx + dy * dy + 0.0f * 0.0f >= 1.0f) continue;
+        break;
+    }
+    v3 rd = mk(dx * c.lr, dy * c.lr, 0.0f * c.lr);
+    v3 off = add(scl(c.un, rd.x), scl(c.vn, rd.y));
+    org = add(c.O, off);
+    dir = sub(add(add(c.W, scl(scl(c.U, u), 0.5f)), scl(scl(c.V, v), 0.5f)), off);
+}
+
+// ---- make_color (RayTracer.cu:16-47) -------------------------------------
+VZ_DEV uint32_t encode8(float c)
+{
+    c = fmaxf(0.0f, fminf(c, 1.0f));
+    float s = c < 0.0031308f ? 12.92f * c : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+    s = fmaxf(0.0f, fminf(s, 1.0f));
+    uint32_t q = (uint32_t)(s * 256.0f);
+    return q < 255u ? q : 255u;
+}
+VZ_DEV uint32_t make_color(float r, float g, float b)
+{
+    return encode8(r) | (encode8(g) << 8) | (encode8(b) << 16) | (255u << 24);
+}
+
+// ---- LBVH node (32 bytes) --------------------------------------------------
+// word0..2: float origin of the node's quantisation grid (its own box min)
+// word3   : biased exponents ex | ey<<8 | ez<<16 | leaf flags<<24
+// word4..6: per axis, bytes lo0 | hi0<<8 | lo1<<16 | hi1<<24 (child boxes
+//           on the grid origin + 2^(e-127) * q)
+// word7   : first child index g; children are g and g+1, each a leaf index
+//           (flag set) or an internal node index
+VZ_DEV float scale_of(uint32_t meta, int k)
+{
+    return __uint_as_float(((meta >> (8 * k)) & 0xFFu) << 23);
+}
+
+}  // namespace vz

@@ -1,0 +1,174 @@
+// vz_engine.h -- the wavefront integrator shared by the plain launch
+// (vzLaunch), the batched form (vzBatchLaunch) and the serving loop.
+//
+// One engine owns the ray queues (two parities), the shade and miss queues,
+// a device table of up to kMaxRequests requests and a scene table.  Each
+// wavefront iteration runs six kernels on one stream:
+//   trace   persistent; pulls 64-ray chunks of the current ray queue, finds
+//           the closest hit, compacts hits into the shade queue and misses
+//           into the miss queue (wave ballot + mbcnt + one atomic per wave)
+//   miss    radiance = throughput * sky, path ends
+//   shade   material scatter; survivors are compacted into the next queue
+//   control one workgroup: finds passes whose paths all ended, plans their
+//           accumulation, grants queue space to requests for new camera rays
+//   apply   adds finished passes into per-pixel sums in sample order and
+//           writes accum/image when a request's last pass lands
+//   regen   camera rays for the granted items into the next queue
+// A "pass" is sample s of every pixel of one request; its radiances land in
+// one of K ring slots, so per-pixel sums are formed in sample order whatever
+// order the paths finish in.  Iterations carry no host round trip; the host
+// only polls a status word between blocks of iterations (or hipGraphs of
+// them).
+#pragma once
+#include <vector>
+
+#include "vz_device.h"
+#include "vz_internal.h"
+
+namespace vz {
+
+struct ReqDesc {             // device table entry, written by the host
+    Cam cam;
+    uint32_t scene;          // index into the scene table
+    uint32_t spp, subframe, max_depth, flags;
+    uint32_t npix, tw, th, tiles_x, npad;
+    uint32_t K;              // pass ring slots
+    uint32_t pad0;
+    unsigned long long total_items;  // spp * npad
+    float* passbuf;          // K * 3 * npix, planar per slot
+    float* sum32;            // 3 * npix
+    double* sum64;           // 3 * npix (FP64 accumulation)
+    uint32_t* camseed;       // npix
+    uint32_t* done;          // K pass counters
+    void* accum;
+    uint32_t* image;
+};
+
+struct ReqState {            // device, advanced by the control kernel
+    unsigned long long next_item;
+    uint32_t acc_next;
+    uint32_t active;
+    uint32_t finish_iter;
+    uint32_t pad;
+    unsigned long long segments;
+};
+
+struct Counters {
+    uint32_t ray_count[2];
+    uint32_t trace_head;
+    uint32_t shade_count;
+    uint32_t miss_count;
+    uint32_t iteration;
+    uint32_t active;          // requests still running
+    uint32_t n_slots;         // request slots in use (table prefix)
+    unsigned long long segments, paths, node_visits, sphere_tests;
+};
+
+struct Plan {
+    uint32_t n_acc, total_acc_pix, n_regen, total_regen;
+    uint32_t acc_req[kMaxRequests];
+    uint32_t acc_first[kMaxRequests];
+    uint32_t acc_cnt[kMaxRequests];
+    uint32_t acc_prefix[kMaxRequests];
+    uint32_t regen_req[kMaxRequests];
+    uint32_t regen_prefix[kMaxRequests];
+    unsigned long long regen_base[kMaxRequests];
+};
+
+struct EngineConfig {
+    uint32_t queue_capacity = 1u << 22;
+    uint32_t tile_w = 8, tile_h = 8;
+    uint32_t iters_per_graph = 16;
+};
+
+struct DeviceBlock {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct Engine {
+    vzContext ctx = nullptr;
+    EngineConfig cfg;
+    uint32_t Q = 0;
+    float4* q_o[2] = {nullptr, nullptr};   // origin xyz, pixel
+    float4* q_d[2] = {nullptr, nullptr};   // dir xyz, seed
+    float4* q_t[2] = {nullptr, nullptr};   // throughput xyz, req<<16 | slot<<8 | depth
+    uint4* shade_q = nullptr;              // ray index, t, leaf
+    uint32_t* miss_q = nullptr;
+    ReqDesc* d_req = nullptr;
+    ReqState* d_state = nullptr;
+    SceneView* d_scenes = nullptr;
+    uint32_t scene_cap = 0;
+    Counters* d_cnt = nullptr;
+    Plan* d_plan = nullptr;
+    Counters* h_cnt = nullptr;             // pinned copy for polling
+    ReqState* h_state = nullptr;           // pinned copy of the request states
+    ReqDesc* h_req = nullptr;              // pinned staging for the table
+    SceneView* h_scenes = nullptr;
+    uint32_t grid_trace = 0, grid_small = 0;
+    hipGraphExec_t graph = nullptr;
+    hipStream_t graph_stream = nullptr;
+    bool graph_count = false;
+    // per-slot buffers, reused when large enough
+    std::vector<DeviceBlock> slot_mem;
+    uint64_t slot_bytes_total = 0;
+    // serving: stream-ordered per-request buffers, slot occupancy, scene table
+    bool stream_alloc = false;
+    std::vector<uint8_t> slot_used;
+    uint32_t n_slots = 0;
+    ReqState* h_state_stage = nullptr;     // pinned staging for state uploads
+    ReqState* h_status = nullptr;          // pinned, 2 x kMaxRequests status copies
+    uint32_t* h_nslots = nullptr;          // pinned staging for the slot count
+    std::vector<const vzScene_st*> scene_list;
+
+    ~Engine();
+};
+
+// Creates an engine for ctx with the given configuration.
+vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out);
+void engine_destroy(Engine* e);
+
+struct RunItem {
+    const vzScene_st* scene;
+    vzLaunchParams params;
+};
+
+struct RunOptions {
+    bool use_graph = false;
+    bool count_visits = false;
+    bool time_trace = false;
+};
+
+struct RunResult {
+    unsigned long long segments = 0, paths = 0, node_visits = 0, sphere_tests = 0;
+    double render_ms = 0.0, trace_ms = 0.0;
+    uint32_t trace_launches = 0, iterations = 0;
+    std::vector<unsigned long long> req_segments;
+    std::vector<uint32_t> req_finish_iter;
+};
+
+// Renders every item to completion (synchronises the stream).
+vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st,
+                    const RunOptions& opt, RunResult* res);
+
+// ---- incremental interface for the serving loop -------------------------
+// Prepares the engine for incremental use (empty table, stream-ordered
+// request buffers, scene table capacity `max_scenes`).
+vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes);
+// Installs one request into a free slot (written to the device tables in
+// stream order, so it joins the wavefront at the next iteration).
+vzStatus engine_admit(Engine* e, const RunItem& item, hipStream_t st, uint32_t* slot);
+// Enqueues one block of wavefront iterations (the captured hipGraph) and a
+// copy of the request states into pinned memory (buffer `k & 1`).
+vzStatus engine_step(Engine* e, hipStream_t st, uint32_t k);
+// Reads slot state from status buffer `k & 1` (valid once the stream passed
+// the engine_step that wrote it).
+void engine_slot_status(Engine* e, uint32_t k, uint32_t slot, bool* active,
+                        unsigned long long* segments);
+// Frees a finished slot's buffers (stream-ordered) and marks it reusable.
+vzStatus engine_release_slot(Engine* e, uint32_t slot, hipStream_t st);
+size_t request_bytes(const vzLaunchParams& p);
+
+Cam make_cam(const vzLaunchParams& p);
+
+}  // namespace vz

@@ -1,0 +1,913 @@
+// vz_engine.hip -- wavefront integrator (see vz_engine.h for the stage
+// list).  Replaces the recursive optixTrace chain of the reference's OptiX
+// programs (Core/RayTracer.cu:163-450) with queue-driven persistent kernels,
+// after Laine, Karras and Aila, "Megakernels Considered Harmful" (HPG 2013).
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "vz_engine.h"
+#include "vz_traverse.h"
+
+namespace vz {
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kStackS = 8;  // LDS stack slots per lane in the trace kernel
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ uint32_t bcast(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Appends the lanes with `pred` to a queue: one atomic per wave, slots in
+// lane order.  Returns the lane's slot (undefined when !pred).
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+{
+    uint64_t m = __ballot(pred);
+    uint32_t n = (uint32_t)__popcll(m);
+    uint32_t base = 0;
+    if (n) {
+        if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1)) base = atomicAdd(counter, n);
+        base = __shfl(base, __ffsll((unsigned long long)m) - 1);
+    }
+    return base + mbcnt(m);
+}
+
+// Adds 1 per lane with `pred` to counter[key]: lanes sharing a key are
+// summed first so each distinct key costs one atomic per wave.
+template <typename F>
+__device__ __forceinline__ void wave_count_by_key(bool pred, uint32_t key, F&& add)
+{
+    uint64_t todo = __ballot(pred);
+    while (todo) {
+        int leader = __ffsll((unsigned long long)todo) - 1;
+        uint32_t lk = __shfl(key, leader);
+        uint64_t same = __ballot(pred && key == lk) & todo;
+        if ((int)__lane_id() == leader) add(lk, (uint32_t)__popcll(same));
+        todo &= ~same;
+    }
+}
+
+__device__ __forceinline__ void terminate(const ReqDesc& D, uint32_t slot, uint32_t pixel, v3 L)
+{
+    float* pb = D.passbuf + (size_t)slot * 3u * D.npix;
+    pb[pixel] = L.x;
+    pb[D.npix + pixel] = L.y;
+    pb[2u * D.npix + pixel] = L.z;
+}
+
+// ---- trace ------------------------------------------------------------------
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlk) k_wf_trace(const SceneView* __restrict__ scenes,
+                                                   const ReqDesc* __restrict__ req,
+                                                   ReqState* __restrict__ st,
+                                                   const float4* __restrict__ q_o,
+                                                   const float4* __restrict__ q_d,
+                                                   const float4* __restrict__ q_t, Counters* cnt,
+                                                   int p, uint4* __restrict__ shade_q,
+                                                   uint32_t* __restrict__ miss_q)
+{
+    __shared__ uint32_t s_node[kStackS * kBlk];
+    __shared__ float s_t[kStackS * kBlk];
+    const uint32_t count = cnt->ray_count[p];
+    const uint32_t lane = __lane_id();
+    uint32_t seg = 0, nv = 0, ns = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&cnt->trace_head, 64u);
+        base = bcast(base);
+        if (base >= count) break;
+        uint32_t i = base + lane;
+        bool valid = i < count;
+        TraceResult r{0.0f, 0xFFFFFFFFu};
+        uint32_t rq = 0;
+        if (valid) {
+            float4 o = q_o[i], d = q_d[i];
+            rq = __float_as_uint(q_t[i].w) >> 16;
+            const SceneView sc = scenes[req[rq].scene];
+            r = closest_hit<kStackS, COUNT>(sc, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e16f,
+                                            s_node + threadIdx.x, s_t + threadIdx.x, kBlk, nv, ns);
+            seg++;
+        }
+        bool hit = valid && r.leaf != 0xFFFFFFFFu;
+        bool miss = valid && !hit;
+        uint32_t hs = wave_append(hit, &cnt->shade_count);
+        if (hit) shade_q[hs] = make_uint4(i, __float_as_uint(r.t), r.leaf, 0u);
+        uint32_t ms = wave_append(miss, &cnt->miss_count);
+        if (miss) miss_q[ms] = i;
+        wave_count_by_key(valid, rq, [&](uint32_t k, uint32_t c) {
+            atomicAdd(&st[k].segments, (unsigned long long)c);
+        });
+    }
+    seg = wave_sum(seg);
+    if (COUNT) {
+        nv = wave_sum(nv);
+        ns = wave_sum(ns);
+    }
+    if (lane == 0 && seg) {
+        atomicAdd(&cnt->segments, (unsigned long long)seg);
+        if (COUNT) {
+            atomicAdd(&cnt->node_visits, (unsigned long long)nv);
+            atomicAdd(&cnt->sphere_tests, (unsigned long long)ns);
+        }
+    }
+}
+
+// ---- miss -------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlk) k_wf_miss(const ReqDesc* __restrict__ req,
+                                                  const float4* __restrict__ q_o,
+                                                  const float4* __restrict__ q_d,
+                                                  const float4* __restrict__ q_t,
+                                                  const Counters* __restrict__ cnt,
+                                                  const uint32_t* __restrict__ miss_q)
+{
+    const uint32_t count = cnt->miss_count;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = (blockIdx.x * kBlk + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * kBlk) >> 6;
+    for (uint32_t base = wave * 64u; base < count; base += nwaves * 64u) {
+        uint32_t j = base + lane;
+        bool valid = j < count;
+        uint32_t key = 0;
+        const ReqDesc* D = nullptr;
+        if (valid) {
+            uint32_t i = miss_q[j];
+            float4 d = q_d[i], t = q_t[i];
+            uint32_t meta = __float_as_uint(t.w);
+            uint32_t rq = meta >> 16, slot = (meta >> 8) & 0xFFu;
+            D = &req[rq];
+            v3 L = mul(mk(t.x, t.y, t.z), sky(mk(d.x, d.y, d.z)));
+            terminate(*D, slot, __float_as_uint(q_o[i].w), L);
+            key = (rq << 8) | slot;
+        }
+        wave_count_by_key(valid, key, [&](uint32_t k, uint32_t c) {
+            atomicAdd(&req[k >> 8].done[k & 0xFFu], c);
+        });
+    }
+}
+
+// ---- shade --------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlk) k_wf_shade(const SceneView* __restrict__ scenes,
+                                                   const ReqDesc* __restrict__ req,
+                                                   const float4* __restrict__ q_o,
+                                                   const float4* __restrict__ q_d,
+                                                   const float4* __restrict__ q_t, Counters* cnt,
+                                                   int p, const uint4* __restrict__ shade_q,
+                                                   float4* __restrict__ n_o, float4* __restrict__ n_d,
+                                                   float4* __restrict__ n_t)
+{
+    const uint32_t count = cnt->shade_count;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = (blockIdx.x * kBlk + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * kBlk) >> 6;
+    uint32_t* next_count = &cnt->ray_count[1 - p];
+    for (uint32_t base = wave * 64u; base < count; base += nwaves * 64u) {
+        uint32_t j = base + lane;
+        bool valid = j < count;
+        bool survive = false, ended = false;
+        uint32_t key = 0, meta = 0, seed = 0, pixel = 0;
+        v3 pos = mk(0.0f, 0.0f, 0.0f), nd = pos, thr = pos;
+        if (valid) {
+            uint4 e = shade_q[j];
+            uint32_t i = e.x;
+            float t = __uint_as_float(e.y);
+            uint32_t leaf = e.z;
+            float4 o4 = q_o[i], d4 = q_d[i], t4 = q_t[i];
+            meta = __float_as_uint(t4.w);
+            uint32_t rq = meta >> 16, slot = (meta >> 8) & 0xFFu, depth = meta & 0xFFu;
+            pixel = __float_as_uint(o4.w);
+            seed = __float_as_uint(d4.w);
+            const ReqDesc* D = &req[rq];
+            key = (rq << 8) | slot;
+            ended = true;
+            v3 L = mk(0.0f, 0.0f, 0.0f);
+            if (depth > 0u) {
+                const SceneView sc = scenes[D->scene];
+                float4 s = sc.spheres[leaf];
+                uint32_t type = sc.leaf_info[leaf] >> 30;
+                float4 mat = sc.mats[leaf];
+                v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+                pos = add(o, scl(d, t));
+                v3 n = scl(sub(pos, mk(s.x, s.y, s.z)), 1.0f / s.w);
+                bool front = dot(d, n) < 0.0f;
+                if (!front) n = neg(n);
+                v3 att;
+                if (scatter(type, mat, d, n, front, seed, nd, att)) {
+                    thr = mul(mk(t4.x, t4.y, t4.z), att);
+                    meta = meta - 1u;
+                    survive = true;
+                    ended = false;
+                }
+            }
+            if (ended) terminate(*D, slot, pixel, L);
+        }
+        uint32_t k = wave_append(survive, next_count);
+        if (survive) {
+            n_o[k] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(pixel));
+            n_d[k] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(seed));
+            n_t[k] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(meta));
+        }
+        wave_count_by_key(ended, key, [&](uint32_t kk, uint32_t c) {
+            atomicAdd(&req[kk >> 8].done[kk & 0xFFu], c);
+        });
+    }
+}
+
+// ---- control (one workgroup of 1024 threads, one per request slot) -------------
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s, uint32_t* total)
+{
+    const uint32_t t = threadIdx.x;
+    s[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        uint32_t x = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    uint32_t incl = s[t];
+    *total = s[1023];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__ req,
+                                                     ReqState* __restrict__ st, Counters* cnt,
+                                                     Plan* __restrict__ plan, uint32_t Q, int p)
+{
+    __shared__ uint32_t s_scan[1024];
+    const uint32_t r = threadIdx.x;
+    const uint32_t nslots = cnt->n_slots;
+    const uint32_t iter = cnt->iteration;
+    uint32_t acc_first = 0, acc_cnt = 0, quota = 0, npix = 0;
+    bool running = false, still = false;
+    unsigned long long next_item = 0;
+    if (r < nslots && st[r].active) {
+        const ReqDesc& D = req[r];
+        ReqState S = st[r];
+        running = true;
+        npix = D.npix;
+        uint32_t a = S.acc_next, c = 0;
+        while (a + c < D.spp && (unsigned long long)(a + c + 1) * D.npad <= S.next_item &&
+               D.done[(a + c) % D.K] == D.npix) {
+            D.done[(a + c) % D.K] = 0u;
+            c++;
+        }
+        acc_first = a;
+        acc_cnt = c;
+        S.acc_next = a + c;
+        if (S.acc_next == D.spp) {
+            S.active = 0u;
+            S.finish_iter = iter;
+        } else {
+            still = true;
+            unsigned long long rem = D.total_items - S.next_item;
+            unsigned long long ring = (unsigned long long)(S.acc_next + D.K) * D.npad - S.next_item;
+            unsigned long long q = rem < ring ? rem : ring;
+            quota = (uint32_t)(q < D.npad ? q : D.npad);
+        }
+        next_item = S.next_item;
+        st[r].acc_next = S.acc_next;
+        st[r].active = S.active;
+        st[r].finish_iter = S.finish_iter;
+    }
+    const uint32_t live = cnt->ray_count[1 - p];
+    const uint32_t freeq = Q > live ? Q - live : 0u;
+    uint32_t tq;
+    uint32_t excl = block_excl_scan(quota, s_scan, &tq);
+    uint32_t grant = excl >= freeq ? 0u : min(quota, freeq - excl);
+    if (running && grant) st[r].next_item = next_item + grant;
+    // compact the accumulation plan
+    uint32_t nA, totA, nR, totR, nact;
+    uint32_t posA = block_excl_scan(acc_cnt ? 1u : 0u, s_scan, &nA);
+    uint32_t pixA = block_excl_scan(acc_cnt ? npix : 0u, s_scan, &totA);
+    uint32_t posR = block_excl_scan(grant ? 1u : 0u, s_scan, &nR);
+    uint32_t preR = block_excl_scan(grant, s_scan, &totR);
+    (void)block_excl_scan(still ? 1u : 0u, s_scan, &nact);
+    if (acc_cnt) {
+        plan->acc_req[posA] = r;
+        plan->acc_first[posA] = acc_first;
+        plan->acc_cnt[posA] = acc_cnt;
+        plan->acc_prefix[posA] = pixA;
+    }
+    if (grant) {
+        plan->regen_req[posR] = r;
+        plan->regen_prefix[posR] = preR;
+        plan->regen_base[posR] = next_item;
+    }
+    if (r == 0) {
+        plan->n_acc = nA;
+        plan->total_acc_pix = totA;
+        plan->n_regen = nR;
+        plan->total_regen = totR;
+        cnt->active = nact;
+        cnt->ray_count[p] = 0u;
+        cnt->trace_head = 0u;
+        cnt->shade_count = 0u;
+        cnt->miss_count = 0u;
+        cnt->iteration = iter + 1u;
+    }
+}
+
+// Largest e with prefix[e] <= j (prefix is ascending, prefix[0] == 0).
+__device__ __forceinline__ uint32_t find_entry(const uint32_t* prefix, uint32_t n, uint32_t j)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= j) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ---- apply: ordered per-pixel accumulation and the final blend/encode --------
+__global__ void __launch_bounds__(kBlk) k_wf_apply(const ReqDesc* __restrict__ req,
+                                                   const Plan* __restrict__ plan)
+{
+    const uint32_t total = plan->total_acc_pix, n = plan->n_acc;
+    for (uint32_t j = blockIdx.x * kBlk + threadIdx.x; j < total; j += gridDim.x * kBlk) {
+        uint32_t e = find_entry(plan->acc_prefix, n, j);
+        const ReqDesc& D = req[plan->acc_req[e]];
+        uint32_t px = j - plan->acc_prefix[e];
+        uint32_t first = plan->acc_first[e], c = plan->acc_cnt[e];
+        bool fin = first + c == D.spp;
+        float oc[3];
+        if (D.flags & VZ_ACCUM_FP64) {
+            double s[3] = {0.0, 0.0, 0.0};
+            if (first) for (int k = 0; k < 3; k++) s[k] = D.sum64[(size_t)k * D.npix + px];
+            for (uint32_t q = 0; q < c; q++) {
+                const float* pb = D.passbuf + (size_t)((first + q) % D.K) * 3u * D.npix;
+                for (int k = 0; k < 3; k++) s[k] += (double)pb[(size_t)k * D.npix + px];
+            }
+            if (!fin) {
+                for (int k = 0; k < 3; k++) D.sum64[(size_t)k * D.npix + px] = s[k];
+                continue;
+            }
+            double* acc = (double*)D.accum + 4 * (size_t)px;
+            double avg[3];
+            for (int k = 0; k < 3; k++) avg[k] = s[k] / (double)D.spp;
+            if (D.subframe > 1u) {
+                double a = 1.0 / (double)D.subframe;
+                for (int k = 0; k < 3; k++) avg[k] = acc[k] + a * (avg[k] - acc[k]);
+            }
+            reinterpret_cast<double2*>(acc)[0] = make_double2(avg[0], avg[1]);
+            reinterpret_cast<double2*>(acc)[1] = make_double2(avg[2], 1.0);
+            for (int k = 0; k < 3; k++) oc[k] = (float)avg[k];
+        } else {
+            float s[3] = {0.0f, 0.0f, 0.0f};
+            if (first) for (int k = 0; k < 3; k++) s[k] = D.sum32[(size_t)k * D.npix + px];
+            for (uint32_t q = 0; q < c; q++) {
+                const float* pb = D.passbuf + (size_t)((first + q) % D.K) * 3u * D.npix;
+                for (int k = 0; k < 3; k++) s[k] = s[k] + pb[(size_t)k * D.npix + px];
+            }
+            if (!fin) {
+                for (int k = 0; k < 3; k++) D.sum32[(size_t)k * D.npix + px] = s[k];
+                continue;
+            }
+            float4* acc = (float4*)D.accum + px;
+            float inv = 1.0f / (float)D.spp;
+            float avg[3] = {s[0] * inv, s[1] * inv, s[2] * inv};
+            if (D.subframe > 1u) {
+                float4 prev = *acc;
+                float a = 1.0f / (float)D.subframe;
+                avg[0] = prev.x + a * (avg[0] - prev.x);
+                avg[1] = prev.y + a * (avg[1] - prev.y);
+                avg[2] = prev.z + a * (avg[2] - prev.z);
+            }
+            *acc = make_float4(avg[0], avg[1], avg[2], 1.0f);
+            for (int k = 0; k < 3; k++) oc[k] = avg[k];
+        }
+        if (D.image) D.image[px] = make_color(oc[0], oc[1], oc[2]);
+    }
+}
+
+// ---- regen: camera rays for the granted items ------------------------------------
+__global__ void __launch_bounds__(kBlk) k_wf_regen(const ReqDesc* __restrict__ req,
+                                                   const Plan* __restrict__ plan, Counters* cnt,
+                                                   int p, float4* __restrict__ n_o,
+                                                   float4* __restrict__ n_d, float4* __restrict__ n_t)
+{
+    const uint32_t total = plan->total_regen, n = plan->n_regen;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = (blockIdx.x * kBlk + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * kBlk) >> 6;
+    uint32_t* next_count = &cnt->ray_count[1 - p];
+    uint32_t paths = 0;
+    for (uint32_t base = wave * 64u; base < total; base += nwaves * 64u) {
+        uint32_t j = base + lane;
+        bool valid = false;
+        v3 o = mk(0.0f, 0.0f, 0.0f), d = o;
+        uint32_t pixel = 0, seed = 0, meta = 0;
+        if (j < total) {
+            uint32_t e = find_entry(plan->regen_prefix, n, j);
+            uint32_t rq = plan->regen_req[e];
+            const ReqDesc& D = req[rq];
+            unsigned long long item = plan->regen_base[e] + (j - plan->regen_prefix[e]);
+            uint32_t s = (uint32_t)(item / D.npad);
+            uint32_t i = (uint32_t)(item % D.npad);
+            uint32_t tsz = D.tw * D.th;
+            uint32_t tile = i / tsz, w = i % tsz;
+            uint32_t x = (tile % D.tiles_x) * D.tw + w % D.tw;
+            uint32_t y = (tile / D.tiles_x) * D.th + w / D.tw;
+            if (x < D.cam.width && y < D.cam.height) {
+                valid = true;
+                pixel = y * D.cam.width + x;
+                seed = s == 0u ? tea4(pixel, D.subframe) : D.camseed[pixel];
+                camera_ray(D.cam, x, y, seed, o, d);
+                D.camseed[pixel] = seed;
+                meta = (rq << 16) | ((s % D.K) << 8) | (D.max_depth - 1u);
+            }
+        }
+        uint32_t k = wave_append(valid, next_count);
+        if (valid) {
+            n_o[k] = make_float4(o.x, o.y, o.z, __uint_as_float(pixel));
+            n_d[k] = make_float4(d.x, d.y, d.z, __uint_as_float(seed));
+            n_t[k] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(meta));
+            paths++;
+        }
+    }
+    paths = wave_sum(paths);
+    if (lane == 0 && paths) atomicAdd(&cnt->paths, (unsigned long long)paths);
+}
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+Cam make_cam(const vzLaunchParams& p)
+{
+    auto nrm = [](v3 v) {
+        float inv = 1.0f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+        v3 r;
+        r.x = v.x * inv;
+        r.y = v.y * inv;
+        r.z = v.z * inv;
+        return r;
+    };
+    Cam c;
+    c.O.x = p.origin[0]; c.O.y = p.origin[1]; c.O.z = p.origin[2];
+    c.U.x = p.u[0]; c.U.y = p.u[1]; c.U.z = p.u[2];
+    c.V.x = p.v[0]; c.V.y = p.v[1]; c.V.z = p.v[2];
+    c.W.x = p.w[0]; c.W.y = p.w[1]; c.W.z = p.w[2];
+    c.un = nrm(c.U);
+    c.vn = nrm(c.V);
+    c.lr = p.lens_radius;
+    c.width = p.width;
+    c.height = p.height;
+    return c;
+}
+
+static uint32_t pass_slots(const vzLaunchParams& p)
+{
+    uint32_t k = p.max_depth + 2u;
+    return std::min<uint32_t>(std::min<uint32_t>(k, p.samples_per_pixel), 255u);
+}
+
+size_t request_bytes(const vzLaunchParams& p)
+{
+    size_t npix = (size_t)p.width * p.height;
+    size_t K = pass_slots(p);
+    bool fp64 = (p.flags & VZ_ACCUM_FP64) != 0;
+    return align_up(K * 3 * npix * 4, 256) + align_up(3 * npix * (fp64 ? 8 : 4), 256) +
+           align_up(npix * 4, 256) + align_up(K * 4, 256) + 1024;
+}
+
+Engine::~Engine()
+{
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_stream) (void)hipStreamDestroy(graph_stream);
+    for (int i = 0; i < 2; i++) {
+        if (q_o[i]) (void)hipFree(q_o[i]);
+        if (q_d[i]) (void)hipFree(q_d[i]);
+        if (q_t[i]) (void)hipFree(q_t[i]);
+    }
+    void* dev[] = {shade_q, miss_q, d_req, d_state, d_scenes, d_cnt, d_plan};
+    for (void* ptr : dev)
+        if (ptr) (void)hipFree(ptr);
+    void* host[] = {h_cnt, h_state, h_req, h_scenes, h_state_stage, h_status, h_nslots};
+    for (void* ptr : host)
+        if (ptr) (void)hipHostFree(ptr);
+    for (auto& b : slot_mem)
+        if (b.ptr) (void)hipFree(b.ptr);
+}
+
+vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
+{
+    Engine* e = new Engine();
+    e->ctx = ctx;
+    e->cfg = cfg;
+    e->Q = std::max<uint32_t>(cfg.queue_capacity, 4096u);
+    e->cfg.queue_capacity = e->Q;
+    if (e->cfg.tile_w == 0) e->cfg.tile_w = 8;
+    if (e->cfg.tile_h == 0) e->cfg.tile_h = 8;
+    if (e->cfg.iters_per_graph < 2) e->cfg.iters_per_graph = 2;
+    e->cfg.iters_per_graph &= ~1u;
+    auto fail = [&](vzStatus s, const char* what) {
+        delete e;
+        return set_error(s, what);
+    };
+    size_t Q = e->Q;
+    for (int i = 0; i < 2; i++) {
+        if (hipMalloc(&e->q_o[i], Q * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&e->q_d[i], Q * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&e->q_t[i], Q * sizeof(float4)) != hipSuccess)
+            return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: ray queue allocation failed");
+    }
+    if (hipMalloc(&e->shade_q, Q * sizeof(uint4)) != hipSuccess ||
+        hipMalloc(&e->miss_q, Q * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&e->d_req, kMaxRequests * sizeof(ReqDesc)) != hipSuccess ||
+        hipMalloc(&e->d_state, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
+        hipMalloc(&e->d_cnt, sizeof(Counters)) != hipSuccess ||
+        hipMalloc(&e->d_plan, sizeof(Plan)) != hipSuccess)
+        return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: table allocation failed");
+    if (hipHostMalloc(&e->h_cnt, 2 * sizeof(Counters)) != hipSuccess ||
+        hipHostMalloc(&e->h_state, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
+        hipHostMalloc(&e->h_req, kMaxRequests * sizeof(ReqDesc)) != hipSuccess)
+        return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: pinned allocation failed");
+    std::memset(e->h_cnt, 0, 2 * sizeof(Counters));
+    std::memset(e->h_state, 0, kMaxRequests * sizeof(ReqState));
+    std::memset(e->h_req, 0, kMaxRequests * sizeof(ReqDesc));
+    if (hipMemset(e->d_state, 0, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
+        hipMemset(e->d_cnt, 0, sizeof(Counters)) != hipSuccess ||
+        hipMemset(e->d_plan, 0, sizeof(Plan)) != hipSuccess)
+        return fail(VZ_ERROR_HIP, "engine: initial memset failed");
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_wf_trace<false>),
+                                                     kBlk, 0) != hipSuccess || occ < 1)
+        occ = 4;
+    uint32_t cus = ctx->num_cus ? ctx->num_cus : 256;
+    e->grid_trace = cus * (uint32_t)occ;
+    e->grid_small = cus * 4;
+    e->slot_mem.resize(kMaxRequests);
+    *out = e;
+    return VZ_SUCCESS;
+}
+
+void engine_destroy(Engine* e) { delete e; }
+
+// Enqueues one wavefront iteration of parity p.
+static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, hipEvent_t ev0,
+                              hipEvent_t ev1)
+{
+    const int n = 1 - p;
+    if (ev0) (void)hipEventRecord(ev0, st);
+    if (count)
+        hipLaunchKernelGGL(k_wf_trace<true>, dim3(e->grid_trace), dim3(kBlk), 0, st, e->d_scenes,
+                           e->d_req, e->d_state, e->q_o[p], e->q_d[p], e->q_t[p], e->d_cnt, p,
+                           e->shade_q, e->miss_q);
+    else
+        hipLaunchKernelGGL(k_wf_trace<false>, dim3(e->grid_trace), dim3(kBlk), 0, st, e->d_scenes,
+                           e->d_req, e->d_state, e->q_o[p], e->q_d[p], e->q_t[p], e->d_cnt, p,
+                           e->shade_q, e->miss_q);
+    if (ev1) (void)hipEventRecord(ev1, st);
+    hipLaunchKernelGGL(k_wf_miss, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_req, e->q_o[p], e->q_d[p],
+                       e->q_t[p], e->d_cnt, e->miss_q);
+    hipLaunchKernelGGL(k_wf_shade, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_scenes, e->d_req,
+                       e->q_o[p], e->q_d[p], e->q_t[p], e->d_cnt, p, e->shade_q, e->q_o[n], e->q_d[n],
+                       e->q_t[n]);
+    hipLaunchKernelGGL(k_wf_control, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt,
+                       e->d_plan, e->Q, p);
+    hipLaunchKernelGGL(k_wf_apply, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_req, e->d_plan);
+    hipLaunchKernelGGL(k_wf_regen, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_req, e->d_plan, e->d_cnt,
+                       p, e->q_o[n], e->q_d[n], e->q_t[n]);
+}
+
+static vzStatus ensure_graph(Engine* e, bool count)
+{
+    if (e->graph && e->graph_count == count) return VZ_SUCCESS;
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    if (!e->graph_stream) VZ_HIP_TRY(hipStreamCreateWithFlags(&e->graph_stream, hipStreamNonBlocking));
+    VZ_HIP_TRY(hipStreamBeginCapture(e->graph_stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t it = 0; it < e->cfg.iters_per_graph; it++)
+        enqueue_iteration(e, e->graph_stream, (int)(it & 1u), count, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    VZ_HIP_TRY(hipStreamEndCapture(e->graph_stream, &g));
+    hipError_t err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err != hipSuccess) return set_error(VZ_ERROR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+    e->graph_count = count;
+    return VZ_SUCCESS;
+}
+
+// Fills slot `slot` of the host table for item `it`; allocates its buffers.
+static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const vzLaunchParams& p,
+                           hipStream_t st)
+{
+    ReqDesc& D = e->h_req[slot];
+    std::memset(&D, 0, sizeof(D));
+    D.cam = make_cam(p);
+    D.scene = scene_idx;
+    D.spp = p.samples_per_pixel;
+    D.subframe = p.subframe_index;
+    D.max_depth = p.max_depth;
+    D.flags = p.flags;
+    D.npix = p.width * p.height;
+    D.tw = e->cfg.tile_w;
+    D.th = e->cfg.tile_h;
+    D.tiles_x = (p.width + D.tw - 1) / D.tw;
+    uint32_t tiles_y = (p.height + D.th - 1) / D.th;
+    D.npad = D.tiles_x * D.tw * tiles_y * D.th;
+    D.K = pass_slots(p);
+    D.total_items = (unsigned long long)D.spp * D.npad;
+    D.accum = p.accum;
+    D.image = (uint32_t*)p.image;
+    size_t need = request_bytes(p);
+    DeviceBlock& b = e->slot_mem[slot];
+    if (e->stream_alloc) {
+        // serving: exact-size buffer from the stream-ordered pool, no device sync
+        if (b.ptr) return set_error(VZ_ERROR_INTERNAL, "slot buffer still held");
+        if (hipMallocAsync(&b.ptr, need, st) != hipSuccess) {
+            b.ptr = nullptr;
+            return set_error(VZ_ERROR_OUT_OF_MEMORY, "engine: request buffers (" + std::to_string(need) +
+                                                         " bytes) do not fit");
+        }
+        b.bytes = need;
+        e->slot_bytes_total += need;
+    } else if (b.bytes < need) {
+        if (b.ptr) {
+            VZ_HIP_TRY(hipStreamSynchronize(st));
+            (void)hipFree(b.ptr);
+            e->slot_bytes_total -= b.bytes;
+            b.ptr = nullptr;
+            b.bytes = 0;
+        }
+        if (hipMalloc(&b.ptr, need) != hipSuccess)
+            return set_error(VZ_ERROR_OUT_OF_MEMORY, "engine: request buffers (" + std::to_string(need) +
+                                                         " bytes) do not fit");
+        b.bytes = need;
+        e->slot_bytes_total += need;
+    }
+    char* base = (char*)b.ptr;
+    size_t npix = D.npix;
+    bool fp64 = (p.flags & VZ_ACCUM_FP64) != 0;
+    D.passbuf = (float*)base;
+    base += align_up((size_t)D.K * 3 * npix * 4, 256);
+    if (fp64) D.sum64 = (double*)base; else D.sum32 = (float*)base;
+    base += align_up(3 * npix * (fp64 ? 8 : 4), 256);
+    D.camseed = (uint32_t*)base;
+    base += align_up(npix * 4, 256);
+    D.done = (uint32_t*)base;
+    VZ_HIP_TRY(hipMemsetAsync(D.done, 0, (size_t)D.K * 4, st));
+    ReqState& S = e->h_state[slot];
+    std::memset(&S, 0, sizeof(S));
+    S.active = 1u;
+    return VZ_SUCCESS;
+}
+
+static vzStatus upload_scenes(Engine* e, const std::vector<const vzScene_st*>& scenes, hipStream_t st)
+{
+    uint32_t ns = (uint32_t)scenes.size();
+    if (ns > e->scene_cap) {
+        if (e->d_scenes) (void)hipFree(e->d_scenes);
+        if (e->h_scenes) (void)hipHostFree(e->h_scenes);
+        e->d_scenes = nullptr;
+        e->h_scenes = nullptr;
+        uint32_t cap = std::max<uint32_t>(ns, 64u);
+        VZ_HIP_TRY(hipMalloc(&e->d_scenes, cap * sizeof(SceneView)));
+        VZ_HIP_TRY(hipHostMalloc(&e->h_scenes, cap * sizeof(SceneView)));
+        e->scene_cap = cap;
+    }
+    for (uint32_t i = 0; i < ns; i++) e->h_scenes[i] = scenes[i]->view();
+    VZ_HIP_TRY(hipMemcpyAsync(e->d_scenes, e->h_scenes, ns * sizeof(SceneView), hipMemcpyHostToDevice, st));
+    return VZ_SUCCESS;
+}
+
+vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st, const RunOptions& opt,
+                    RunResult* res)
+{
+    uint32_t n = (uint32_t)items.size();
+    if (n == 0) return VZ_SUCCESS;
+    if (n > kMaxRequests) return set_error(VZ_ERROR_CAPACITY, "too many requests in one batch");
+    // the whole table is reset: wait for anything still using it
+    VZ_HIP_TRY(hipStreamSynchronize(st));
+    std::vector<const vzScene_st*> scenes;
+    std::vector<uint32_t> scene_idx(n);
+    for (uint32_t i = 0; i < n; i++) {
+        auto it = std::find(scenes.begin(), scenes.end(), items[i].scene);
+        if (it == scenes.end()) {
+            scene_idx[i] = (uint32_t)scenes.size();
+            scenes.push_back(items[i].scene);
+        } else {
+            scene_idx[i] = (uint32_t)(it - scenes.begin());
+        }
+    }
+    VZ_TRY(upload_scenes(e, scenes, st));
+    for (uint32_t i = 0; i < n; i++) VZ_TRY(setup_slot(e, i, scene_idx[i], items[i].params, st));
+    VZ_HIP_TRY(hipMemcpyAsync(e->d_req, e->h_req, n * sizeof(ReqDesc), hipMemcpyHostToDevice, st));
+    VZ_HIP_TRY(hipMemcpyAsync(e->d_state, e->h_state, n * sizeof(ReqState), hipMemcpyHostToDevice, st));
+    Counters c0;
+    std::memset(&c0, 0, sizeof(c0));
+    c0.n_slots = n;
+    c0.active = n;
+    e->h_cnt[0] = c0;
+    VZ_HIP_TRY(hipMemcpyAsync(e->d_cnt, &e->h_cnt[0], sizeof(Counters), hipMemcpyHostToDevice, st));
+    VZ_HIP_TRY(hipStreamSynchronize(st));  // staging buffers are reused below
+
+    const bool use_graph = opt.use_graph && !opt.time_trace;
+    if (use_graph) VZ_TRY(ensure_graph(e, opt.count_visits));
+    const uint32_t B = e->cfg.iters_per_graph;
+    hipEvent_t t0, t1, blk_ev[2];
+    VZ_HIP_TRY(hipEventCreate(&t0));
+    VZ_HIP_TRY(hipEventCreate(&t1));
+    VZ_HIP_TRY(hipEventCreateWithFlags(&blk_ev[0], hipEventDisableTiming));
+    VZ_HIP_TRY(hipEventCreateWithFlags(&blk_ev[1], hipEventDisableTiming));
+    std::vector<hipEvent_t> tev[2];
+    if (opt.time_trace) {
+        for (int k = 0; k < 2; k++) {
+            tev[k].resize(2 * B);
+            for (auto& ev : tev[k]) VZ_HIP_TRY(hipEventCreate(&ev));
+        }
+    }
+    double trace_ms = 0.0;
+    uint32_t trace_launches = 0;
+    auto launch_block = [&](uint32_t k) -> vzStatus {
+        if (use_graph) {
+            VZ_HIP_TRY(hipGraphLaunch(e->graph, st));
+        } else {
+            for (uint32_t it = 0; it < B; it++) {
+                hipEvent_t a = opt.time_trace ? tev[k & 1][2 * it] : nullptr;
+                hipEvent_t b = opt.time_trace ? tev[k & 1][2 * it + 1] : nullptr;
+                enqueue_iteration(e, st, (int)(it & 1u), opt.count_visits, a, b);
+            }
+            VZ_HIP_TRY(hipGetLastError());
+        }
+        VZ_HIP_TRY(hipMemcpyAsync(&e->h_cnt[k & 1], e->d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
+        VZ_HIP_TRY(hipEventRecord(blk_ev[k & 1], st));
+        return VZ_SUCCESS;
+    };
+    auto harvest = [&](uint32_t k) -> vzStatus {
+        VZ_HIP_TRY(hipEventSynchronize(blk_ev[k & 1]));
+        if (opt.time_trace) {
+            for (uint32_t it = 0; it < B; it++) {
+                float ms = 0.0f;
+                VZ_HIP_TRY(hipEventElapsedTime(&ms, tev[k & 1][2 * it], tev[k & 1][2 * it + 1]));
+                trace_ms += ms;
+                trace_launches++;
+            }
+        }
+        return VZ_SUCCESS;
+    };
+    VZ_HIP_TRY(hipEventRecord(t0, st));
+    vzStatus rs = launch_block(0);
+    uint32_t k = 0;
+    const uint32_t max_blocks = 1u << 20;
+    while (rs == VZ_SUCCESS) {
+        rs = launch_block(k + 1);
+        if (rs != VZ_SUCCESS) break;
+        rs = harvest(k);
+        if (rs != VZ_SUCCESS) break;
+        bool done = e->h_cnt[k & 1].active == 0 && e->h_cnt[k & 1].iteration > 0;
+        k++;
+        if (done) {
+            rs = harvest(k);
+            break;
+        }
+        if (k >= max_blocks) {
+            rs = set_error(VZ_ERROR_INTERNAL, "wavefront did not converge");
+            break;
+        }
+    }
+    if (rs == VZ_SUCCESS) {
+        VZ_HIP_TRY(hipEventRecord(t1, st));
+        VZ_HIP_TRY(hipEventSynchronize(t1));
+    } else {
+        (void)hipStreamSynchronize(st);
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipEventDestroy(blk_ev[0]);
+    (void)hipEventDestroy(blk_ev[1]);
+    for (int q = 0; q < 2; q++)
+        for (auto& ev : tev[q]) (void)hipEventDestroy(ev);
+    VZ_TRY(rs);
+    Counters fin;
+    VZ_HIP_TRY(hipMemcpy(&fin, e->d_cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+    VZ_HIP_TRY(hipMemcpy(e->h_state, e->d_state, n * sizeof(ReqState), hipMemcpyDeviceToHost));
+    if (res) {
+        res->segments = fin.segments;
+        res->paths = fin.paths;
+        res->node_visits = fin.node_visits;
+        res->sphere_tests = fin.sphere_tests;
+        res->render_ms = ms;
+        res->trace_ms = trace_ms;
+        res->trace_launches = trace_launches;
+        res->iterations = fin.iteration;
+        res->req_segments.resize(n);
+        res->req_finish_iter.resize(n);
+        for (uint32_t i = 0; i < n; i++) {
+            res->req_segments[i] = e->h_state[i].segments;
+            res->req_finish_iter[i] = e->h_state[i].finish_iter;
+        }
+    }
+    return VZ_SUCCESS;
+}
+
+// ---- serving -----------------------------------------------------------------------
+vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes)
+{
+    e->stream_alloc = true;
+    e->slot_used.assign(kMaxRequests, 0);
+    e->n_slots = 0;
+    e->scene_list.clear();
+    max_scenes = std::max<uint32_t>(max_scenes, 1u);
+    if (e->scene_cap < max_scenes) {
+        if (e->d_scenes) (void)hipFree(e->d_scenes);
+        if (e->h_scenes) (void)hipHostFree(e->h_scenes);
+        e->d_scenes = nullptr;
+        e->h_scenes = nullptr;
+        VZ_HIP_TRY(hipMalloc(&e->d_scenes, max_scenes * sizeof(SceneView)));
+        VZ_HIP_TRY(hipHostMalloc(&e->h_scenes, max_scenes * sizeof(SceneView)));
+        e->scene_cap = max_scenes;
+    }
+    if (!e->h_state_stage) VZ_HIP_TRY(hipHostMalloc(&e->h_state_stage, kMaxRequests * sizeof(ReqState)));
+    if (!e->h_status) VZ_HIP_TRY(hipHostMalloc(&e->h_status, 2 * kMaxRequests * sizeof(ReqState)));
+    if (!e->h_nslots) VZ_HIP_TRY(hipHostMalloc(&e->h_nslots, 64));
+    std::memset(e->h_status, 0, 2 * kMaxRequests * sizeof(ReqState));
+    VZ_HIP_TRY(hipMemsetAsync(e->d_state, 0, kMaxRequests * sizeof(ReqState), st));
+    VZ_HIP_TRY(hipMemsetAsync(e->d_cnt, 0, sizeof(Counters), st));
+    VZ_TRY(ensure_graph(e, false));
+    VZ_HIP_TRY(hipStreamSynchronize(st));
+    return VZ_SUCCESS;
+}
+
+vzStatus engine_admit(Engine* e, const RunItem& item, hipStream_t st, uint32_t* slot_out)
+{
+    uint32_t slot = kMaxRequests;
+    for (uint32_t i = 0; i < kMaxRequests; i++)
+        if (!e->slot_used[i]) {
+            slot = i;
+            break;
+        }
+    if (slot == kMaxRequests) return set_error(VZ_ERROR_CAPACITY, "request table full");
+    uint32_t sidx;
+    auto it = std::find(e->scene_list.begin(), e->scene_list.end(), item.scene);
+    if (it == e->scene_list.end()) {
+        if (e->scene_list.size() >= e->scene_cap) return set_error(VZ_ERROR_CAPACITY, "scene table full");
+        sidx = (uint32_t)e->scene_list.size();
+        e->scene_list.push_back(item.scene);
+        e->h_scenes[sidx] = item.scene->view();
+        VZ_HIP_TRY(hipMemcpyAsync(e->d_scenes + sidx, e->h_scenes + sidx, sizeof(SceneView),
+                                  hipMemcpyHostToDevice, st));
+    } else {
+        sidx = (uint32_t)(it - e->scene_list.begin());
+    }
+    VZ_TRY(setup_slot(e, slot, sidx, item.params, st));
+    e->h_state_stage[slot] = e->h_state[slot];
+    VZ_HIP_TRY(hipMemcpyAsync(e->d_req + slot, e->h_req + slot, sizeof(ReqDesc), hipMemcpyHostToDevice, st));
+    VZ_HIP_TRY(hipMemcpyAsync(e->d_state + slot, e->h_state_stage + slot, sizeof(ReqState),
+                              hipMemcpyHostToDevice, st));
+    e->slot_used[slot] = 1;
+    if (slot + 1 > e->n_slots) {
+        e->n_slots = slot + 1;
+        // the slot count only grows while serving, so a later value read by
+        // an earlier copy is still correct
+        *e->h_nslots = e->n_slots;
+        VZ_HIP_TRY(hipMemcpyAsync(&e->d_cnt->n_slots, e->h_nslots, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    }
+    *slot_out = slot;
+    return VZ_SUCCESS;
+}
+
+vzStatus engine_step(Engine* e, hipStream_t st, uint32_t k)
+{
+    VZ_HIP_TRY(hipGraphLaunch(e->graph, st));
+    if (e->n_slots)
+        VZ_HIP_TRY(hipMemcpyAsync(e->h_status + (k & 1u) * kMaxRequests, e->d_state, e->n_slots * sizeof(ReqState),
+                                  hipMemcpyDeviceToHost, st));
+    return VZ_SUCCESS;
+}
+
+void engine_slot_status(Engine* e, uint32_t k, uint32_t slot, bool* active, unsigned long long* segments)
+{
+    const ReqState& s = e->h_status[(k & 1u) * kMaxRequests + slot];
+    *active = s.active != 0;
+    *segments = s.segments;
+}
+
+vzStatus engine_release_slot(Engine* e, uint32_t slot, hipStream_t st)
+{
+    DeviceBlock& b = e->slot_mem[slot];
+    if (b.ptr) {
+        VZ_HIP_TRY(hipFreeAsync(b.ptr, st));
+        e->slot_bytes_total -= b.bytes;
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    e->slot_used[slot] = 0;
+    return VZ_SUCCESS;
+}
+
+}  // namespace vz

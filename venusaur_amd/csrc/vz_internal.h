@@ -1,0 +1,117 @@
+// vz_internal.h -- host-side internals of libvz shared across translation
+// units: context, scene, error reporting, kernel-launch entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vz_api.h"
+
+namespace vz {
+
+// Largest BVH depth the traversal stack supports (levels from the root to
+// the deepest leaf).  Karras trees over 30-bit codes with index tie-breaks
+// stay well below this for any scene that fits in memory.
+constexpr uint32_t kMaxDepth = 64;
+constexpr uint32_t kMaxRequests = 1024;  // device request-table size
+
+// Records a message for vzGetLastError and returns the status.
+vzStatus set_error(vzStatus s, const std::string& msg);
+
+#define VZ_HIP_TRY(expr)                                                              \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::vz::set_error(VZ_ERROR_HIP, std::string(#expr) + ": " +          \
+                                                     hipGetErrorString(e_));          \
+    } while (0)
+
+#define VZ_TRY(expr)                                  \
+    do {                                              \
+        vzStatus s_ = (expr);                         \
+        if (s_ != VZ_SUCCESS) return s_;              \
+    } while (0)
+
+struct SceneView {           // what kernels need to traverse one scene
+    const uint4* nodes;      // 2 uint4 per node
+    const float4* spheres;   // leaf order {c, r}
+    const uint32_t* leaf_info;  // leaf -> prim id | material type << 30
+    const float4* mats;      // leaf order {albedo rgb, fuzz | ior}
+    uint32_t n;
+    uint32_t pad;
+};
+
+struct Engine;
+
+}  // namespace vz
+
+struct vzContext_st {
+    int device = 0;
+    uint32_t num_cus = 0;
+    uint64_t hbm_bytes = 0;
+    std::string name;
+    bool tuned = false;
+    vzTuning tuning{};
+    std::mutex mu;
+    vz::Engine* engine = nullptr;  // lazily created wavefront engine
+};
+
+struct vzScene_st {
+    vzContext ctx = nullptr;
+    uint32_t n = 0;
+    // inputs (original order)
+    float4* d_sph_in = nullptr;
+    vzMaterial* d_mat_in = nullptr;
+    // build scratch
+    uint32_t* d_keys[2] = {nullptr, nullptr};
+    uint32_t* d_vals[2] = {nullptr, nullptr};
+    uint32_t* d_hist = nullptr;
+    float* d_bounds = nullptr;        // partial min/max per block + final 9 floats
+    uint32_t hist_blocks = 0;
+    // result (leaf order)
+    float4* d_sph = nullptr;
+    uint32_t* d_leaf_info = nullptr;
+    float4* d_mat = nullptr;
+    uint32_t* d_morton = nullptr;     // sorted keys (points into d_keys[0])
+    uint32_t* d_leaf_prim = nullptr;  // sorted vals (points into d_vals[0])
+    uint4* d_nodes = nullptr;
+    float* d_boxes = nullptr;
+    uint32_t* d_child = nullptr;      // g | flags << 30
+    uint32_t* d_parent_int = nullptr;
+    uint32_t* d_parent_leaf = nullptr;
+    uint32_t* d_visit = nullptr;
+    uint32_t* d_height = nullptr;
+    uint32_t depth = 0;
+    uint64_t device_bytes = 0;
+    double build_ms = 0.0;
+    vz::SceneView view() const
+    {
+        vz::SceneView v;
+        v.nodes = d_nodes;
+        v.spheres = d_sph;
+        v.leaf_info = d_leaf_info;
+        v.mats = d_mat;
+        v.n = n;
+        v.pad = 0;
+        return v;
+    }
+};
+
+namespace vz {
+// vz_lbvh.hip
+vzStatus lbvh_alloc(vzScene s);
+vzStatus lbvh_build(vzScene s, hipStream_t st);   // full build from d_sph_in
+vzStatus lbvh_refit(vzScene s, hipStream_t st);   // same topology, new d_sph_in
+void lbvh_free(vzScene s);
+
+// vz_trace.hip
+vzStatus trace_rays(const SceneView& sc, const vzRay* rays, uint32_t n, vzHit* hits,
+                    hipStream_t st, bool count, unsigned long long* d_counters);
+
+// vz_megakernel.hip
+vzStatus megakernel_launch(const SceneView& sc, const vzLaunchParams& p, hipStream_t st,
+                           unsigned long long* d_counters, bool count);
+}  // namespace vz

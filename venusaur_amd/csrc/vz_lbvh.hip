@@ -1,0 +1,544 @@
+// vz_lbvh.hip -- LBVH build on the GPU, replacing optixAccelBuild over
+// custom-primitive AABBs (Core/Renderer.h:160-255).
+//
+// Published methods: Lauterbach et al. 2009 (LBVH from Morton codes),
+// Karras 2012 "Maximizing Parallelism in the Construction of BVHs, Octrees,
+// and k-d Trees" (parent linking, atomic-flag bottom-up boxes).  The output
+// is bit-identical to the CPU build in oracle/: same Morton quantisation,
+// stable sort, Karras topology with index tie-breaks, min/max boxes and node
+// quantisation.
+//
+// Stages (all on one stream):
+//   1. centre bounds       two-pass min/max reduction
+//   2. Morton codes        10 bits per axis of the centre in the bounds
+//   3. radix sort          4 passes of 8-bit digits; per-tile histograms,
+//                          one scan, stable scatter ranked with wave ballots
+//                          (match-any over the digit bits + mbcnt) and staged
+//                          through LDS so global writes are runs per digit
+//   4. leaf reorder        spheres/materials into Morton (leaf) order
+//   5. Karras linking      one thread per internal node
+//   6. refit + pack        one thread per leaf walks up; the second arrival at
+//                          a node computes its box and writes the 32-byte node
+#include "vz_device.h"
+#include "vz_internal.h"
+
+namespace vz {
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlk * kItems;  // keys per radix tile
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// ---- 1. bounds ---------------------------------------------------------------
+__global__ void __launch_bounds__(kBlk) k_bounds_partial(const float4* __restrict__ sph, uint32_t n,
+                                                         float* __restrict__ partial)
+{
+    __shared__ float s[6][kBlk];
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (uint32_t i = blockIdx.x * kBlk + threadIdx.x; i < n; i += gridDim.x * kBlk) {
+        float4 c = sph[i];
+        lo[0] = fminf(lo[0], c.x); lo[1] = fminf(lo[1], c.y); lo[2] = fminf(lo[2], c.z);
+        hi[0] = fmaxf(hi[0], c.x); hi[1] = fmaxf(hi[1], c.y); hi[2] = fmaxf(hi[2], c.z);
+    }
+    for (int k = 0; k < 3; k++) { s[k][threadIdx.x] = lo[k]; s[3 + k][threadIdx.x] = hi[k]; }
+    __syncthreads();
+    for (int w = kBlk / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 3; k++) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// out: lo[3], hi[3], scale[3] with scale = ext > 0 ? 1024/ext : 0
+__global__ void k_bounds_final(const float* __restrict__ partial, uint32_t nb, float* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    float lo[3] = {partial[0], partial[1], partial[2]}, hi[3] = {partial[3], partial[4], partial[5]};
+    for (uint32_t b = 1; b < nb; b++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], partial[b * 6 + k]);
+            hi[k] = fmaxf(hi[k], partial[b * 6 + 3 + k]);
+        }
+    for (int k = 0; k < 3; k++) {
+        float ext = hi[k] - lo[k];
+        out[k] = lo[k];
+        out[3 + k] = hi[k];
+        out[6 + k] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+    }
+}
+
+// ---- 2. Morton ---------------------------------------------------------------
+__device__ __forceinline__ uint32_t expand10(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void __launch_bounds__(kBlk) k_morton(const float4* __restrict__ sph, uint32_t n,
+                                                 const float* __restrict__ bnd,
+                                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    float4 c = sph[i];
+    float cc[3] = {c.x, c.y, c.z};
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        float f = (cc[k] - bnd[k]) * bnd[6 + k];
+        int qi = (int)f;
+        qi = qi < 0 ? 0 : (qi > 1023 ? 1023 : qi);
+        q[k] = (uint32_t)qi;
+    }
+    keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    vals[i] = i;
+}
+
+// ---- 3. radix sort -------------------------------------------------------------
+__global__ void __launch_bounds__(kBlk) k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                     int shift, uint32_t nblocks,
+                                                     uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kTile;
+    for (int r = 0; r < kItems; r++) {
+        uint32_t i = base + r * kBlk + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of `total` words in place, one block of 1024 threads.
+__global__ void __launch_bounds__(1024) k_scan_excl(uint32_t* __restrict__ a, uint32_t total)
+{
+    __shared__ uint32_t part[1024];
+    uint32_t chunk = (total + 1023) / 1024;
+    uint32_t b = threadIdx.x * chunk, e = min(b + chunk, total);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; i++) s += a[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (uint32_t i = b; i < e; i++) {
+        uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                        const uint32_t* __restrict__ vin, uint32_t n,
+                                                        int shift, uint32_t nblocks,
+                                                        const uint32_t* __restrict__ hist,
+                                                        uint32_t* __restrict__ kout,
+                                                        uint32_t* __restrict__ vout)
+{
+    __shared__ uint32_t s_count[256];   // running per-digit count in this tile
+    __shared__ uint32_t s_start[256];   // tile-local start of each digit
+    __shared__ uint32_t s_wcnt[4][256];
+    __shared__ uint32_t s_wpre[4][256];
+    __shared__ uint32_t s_key[kTile];
+    __shared__ uint32_t s_val[kTile];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+    s_count[tid] = 0;
+    for (int w = 0; w < 4; w++) s_wcnt[w][tid] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kTile;
+    uint32_t key[kItems], val[kItems], rank[kItems];
+    for (int r = 0; r < kItems; r++) {
+        uint32_t i = base + r * kBlk + tid;
+        bool valid = i < n;
+        key[r] = valid ? kin[i] : 0u;
+        val[r] = valid ? vin[i] : 0u;
+        uint32_t d = (key[r] >> shift) & 0xFFu;
+        // lanes of this wave holding the same digit (match-any over 8 bits)
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            uint64_t m = __ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? m : ~m;
+        }
+        uint32_t rk = mbcnt(peers);
+        if (valid && (uint32_t)(__ffsll((unsigned long long)peers) - 1) == lane)
+            s_wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {   // thread t owns digit t: prefix over the 4 waves, in wave order
+            uint32_t run = s_count[tid];
+            for (int w = 0; w < 4; w++) {
+                uint32_t c = s_wcnt[w][tid];
+                s_wpre[w][tid] = run;
+                run += c;
+                s_wcnt[w][tid] = 0;
+            }
+            s_count[tid] = run;
+        }
+        __syncthreads();
+        rank[r] = valid ? s_wpre[wave][d] + rk : 0xFFFFFFFFu;
+        __syncthreads();
+    }
+    // exclusive scan of the tile's digit counts (256 entries, one per thread)
+    {
+        uint32_t v = s_count[tid];
+        s_start[tid] = v;
+        __syncthreads();
+        for (uint32_t off = 1; off < 256; off <<= 1) {
+            uint32_t t = tid >= off ? s_start[tid - off] : 0u;
+            __syncthreads();
+            s_start[tid] += t;
+            __syncthreads();
+        }
+        s_start[tid] -= v;
+        __syncthreads();
+    }
+    for (int r = 0; r < kItems; r++) {
+        if (rank[r] != 0xFFFFFFFFu) {
+            uint32_t d = (key[r] >> shift) & 0xFFu;
+            uint32_t pos = s_start[d] + rank[r];
+            s_key[pos] = key[r];
+            s_val[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    uint32_t cnt = n - base < (uint32_t)kTile ? n - base : (uint32_t)kTile;
+    for (uint32_t i = tid; i < cnt; i += kBlk) {
+        uint32_t k = s_key[i];
+        uint32_t d = (k >> shift) & 0xFFu;
+        uint32_t g = hist[d * nblocks + blockIdx.x] + (i - s_start[d]);
+        kout[g] = k;
+        vout[g] = s_val[i];
+    }
+}
+
+// ---- 4. leaf reorder -----------------------------------------------------------
+__global__ void __launch_bounds__(kBlk) k_reorder(const float4* __restrict__ sph_in,
+                                                  const vzMaterial* __restrict__ mat_in,
+                                                  const uint32_t* __restrict__ perm, uint32_t n,
+                                                  float4* __restrict__ sph, uint32_t* __restrict__ info,
+                                                  float4* __restrict__ mat)
+{
+    uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = perm[i];
+    sph[i] = sph_in[p];
+    vzMaterial m = mat_in[p];
+    uint32_t type = m.type > 2u ? 0u : m.type;
+    info[i] = p | (type << 30);
+    float param = type == 1u ? m.fuzz : (type == 2u ? m.ir : 0.0f);
+    mat[i] = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], param);
+}
+
+// refit reads spheres in leaf order but geometry updates arrive in prim order
+__global__ void __launch_bounds__(kBlk) k_reorder_sph(const float4* __restrict__ sph_in,
+                                                      const uint32_t* __restrict__ perm, uint32_t n,
+                                                      float4* __restrict__ sph)
+{
+    uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) sph[i] = sph_in[perm[i]];
+}
+
+// ---- 5. Karras linking -----------------------------------------------------------
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ k, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+    return __clz((int)(a ^ b));
+}
+
+__global__ void __launch_bounds__(kBlk) k_karras(const uint32_t* __restrict__ K, uint32_t n_,
+                                                 uint32_t* __restrict__ child,
+                                                 uint32_t* __restrict__ parent_int,
+                                                 uint32_t* __restrict__ parent_leaf,
+                                                 uint32_t* __restrict__ visit)
+{
+    int n = (int)n_;
+    int i = (int)(blockIdx.x * kBlk + threadIdx.x);
+    if (i >= n - 1) return;
+    visit[i] = 0u;
+    if (i == 0) parent_int[0] = 0xFFFFFFFFu;
+    int d = (kdelta(K, n, i, i + 1) - kdelta(K, n, i, i - 1)) > 0 ? 1 : -1;
+    int dmin = kdelta(K, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(K, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (kdelta(K, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = kdelta(K, n, i, j);
+    int s = 0, div = 2;
+    int t = (l + div - 1) / div;
+    for (;;) {
+        if (kdelta(K, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t == 1) break;
+        div *= 2;
+        t = (l + div - 1) / div;
+    }
+    int g = i + s * d + (d < 0 ? d : 0);
+    int mn = min(i, j), mx = max(i, j);
+    uint32_t f = 0;
+    if (mn == g) { f |= 1u; parent_leaf[g] = (uint32_t)i; } else { parent_int[g] = (uint32_t)i; }
+    if (mx == g + 1) { f |= 2u; parent_leaf[g + 1] = (uint32_t)i; } else { parent_int[g + 1] = (uint32_t)i; }
+    child[i] = (uint32_t)g | (f << 30);
+}
+
+// ---- 6. refit + pack ---------------------------------------------------------------
+__device__ __forceinline__ uint32_t pick_exp(float lo, float hi)
+{
+    float ext = hi - lo;
+    uint32_t e = 1;
+    if (ext > 0.0f) {
+        float m = ext / 255.0f;
+        uint32_t b = __float_as_uint(m);
+        uint32_t ex = b >> 23, mant = b & 0x7FFFFFu;
+        e = (ex == 0u) ? 1u : ex + (mant != 0u ? 1u : 0u);
+        if (e < 1u) e = 1u;
+    }
+    while (e < 254u && __builtin_fmaf(__uint_as_float(e << 23), 255.0f, lo) < hi) e++;
+    return e;
+}
+
+__device__ __forceinline__ uint32_t quant_lo(float v, float lo, float s)
+{
+    int q = (int)floorf((v - lo) / s);
+    q = q < 0 ? 0 : (q > 255 ? 255 : q);
+    while (q > 0 && __builtin_fmaf(s, (float)q, lo) > v) q--;
+    return (uint32_t)q;
+}
+
+__device__ __forceinline__ uint32_t quant_hi(float v, float lo, float s)
+{
+    int q = (int)ceilf((v - lo) / s);
+    q = q < 0 ? 0 : (q > 255 ? 255 : q);
+    while (q < 255 && __builtin_fmaf(s, (float)q, lo) < v) q++;
+    return (uint32_t)q;
+}
+
+__device__ __forceinline__ float ld_relaxed(const float* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_relaxed_u(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(kBlk) k_refit(const float4* __restrict__ sph, uint32_t n,
+                                                const uint32_t* __restrict__ child,
+                                                const uint32_t* __restrict__ parent_int,
+                                                const uint32_t* __restrict__ parent_leaf,
+                                                uint32_t* visit, float* boxes, uint32_t* height,
+                                                uint4* __restrict__ nodes)
+{
+    uint32_t leaf = blockIdx.x * kBlk + threadIdx.x;
+    if (leaf >= n) return;
+    uint32_t p = parent_leaf[leaf];
+    for (;;) {
+        __threadfence();
+        uint32_t old = __hip_atomic_fetch_add(&visit[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0u) return;
+        __threadfence();
+        uint32_t cw = child[p];
+        uint32_t g = cw & 0x3FFFFFFFu, flags = cw >> 30;
+        float cb[2][6];
+        uint32_t h = 0;
+        for (int c = 0; c < 2; c++) {
+            uint32_t ch = g + (uint32_t)c;
+            if ((flags >> c) & 1u) {
+                float4 s = sph[ch];
+                float r = fabsf(s.w);
+                cb[c][0] = s.x - r; cb[c][1] = s.y - r; cb[c][2] = s.z - r;
+                cb[c][3] = s.x + r; cb[c][4] = s.y + r; cb[c][5] = s.z + r;
+            } else {
+                for (int k = 0; k < 6; k++) cb[c][k] = ld_relaxed(&boxes[ch * 6 + k]);
+                h = max(h, ld_relaxed_u(&height[ch]));
+            }
+        }
+        float nb[6];
+        for (int k = 0; k < 3; k++) {
+            nb[k] = fminf(cb[0][k], cb[1][k]);
+            nb[3 + k] = fmaxf(cb[0][3 + k], cb[1][3 + k]);
+        }
+        for (int k = 0; k < 6; k++)
+            __hip_atomic_store(&boxes[p * 6 + k], nb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&height[p], h + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t meta = 0, q[3];
+        float o[3];
+        for (int k = 0; k < 3; k++) {
+            float lo = nb[k], hi = nb[3 + k];
+            uint32_t e = pick_exp(lo, hi);
+            float s = __uint_as_float(e << 23);
+            meta |= e << (8 * k);
+            uint32_t l0 = quant_lo(cb[0][k], lo, s), h0 = quant_hi(cb[0][3 + k], lo, s);
+            uint32_t l1 = quant_lo(cb[1][k], lo, s), h1 = quant_hi(cb[1][3 + k], lo, s);
+            o[k] = lo;
+            q[k] = l0 | (h0 << 8) | (l1 << 16) | (h1 << 24);
+        }
+        meta |= flags << 24;
+        nodes[2 * p] = make_uint4(__float_as_uint(o[0]), __float_as_uint(o[1]), __float_as_uint(o[2]), meta);
+        nodes[2 * p + 1] = make_uint4(q[0], q[1], q[2], g);
+        if (p == 0u) return;
+        p = parent_int[p];
+    }
+}
+
+__global__ void k_zero(uint32_t* a, uint32_t n)
+{
+    uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) a[i] = 0u;
+}
+
+inline uint32_t blocks_for(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
+
+}  // namespace
+
+vzStatus lbvh_alloc(vzScene s)
+{
+    uint32_t n = s->n, ni = n > 1 ? n - 1 : 1;
+    s->hist_blocks = blocks_for(n, kTile);
+    uint64_t bytes = 0;
+    auto al = [&](void** p, size_t b) -> vzStatus {
+        hipError_t e = hipMalloc(p, b);
+        if (e != hipSuccess) return set_error(VZ_ERROR_OUT_OF_MEMORY, "hipMalloc of scene buffers failed");
+        bytes += b;
+        return VZ_SUCCESS;
+    };
+    VZ_TRY(al((void**)&s->d_sph_in, sizeof(float4) * n));
+    VZ_TRY(al((void**)&s->d_mat_in, sizeof(vzMaterial) * n));
+    for (int i = 0; i < 2; i++) {
+        VZ_TRY(al((void**)&s->d_keys[i], sizeof(uint32_t) * n));
+        VZ_TRY(al((void**)&s->d_vals[i], sizeof(uint32_t) * n));
+    }
+    VZ_TRY(al((void**)&s->d_hist, sizeof(uint32_t) * 256 * s->hist_blocks));
+    VZ_TRY(al((void**)&s->d_bounds, sizeof(float) * (6 * 1024 + 16)));
+    VZ_TRY(al((void**)&s->d_sph, sizeof(float4) * n));
+    VZ_TRY(al((void**)&s->d_leaf_info, sizeof(uint32_t) * n));
+    VZ_TRY(al((void**)&s->d_mat, sizeof(float4) * n));
+    VZ_TRY(al((void**)&s->d_nodes, sizeof(uint4) * 2 * ni));
+    VZ_TRY(al((void**)&s->d_boxes, sizeof(float) * 6 * ni));
+    VZ_TRY(al((void**)&s->d_child, sizeof(uint32_t) * ni));
+    VZ_TRY(al((void**)&s->d_parent_int, sizeof(uint32_t) * ni));
+    VZ_TRY(al((void**)&s->d_parent_leaf, sizeof(uint32_t) * n));
+    VZ_TRY(al((void**)&s->d_visit, sizeof(uint32_t) * ni));
+    VZ_TRY(al((void**)&s->d_height, sizeof(uint32_t) * ni));
+    s->device_bytes = bytes;
+    return VZ_SUCCESS;
+}
+
+void lbvh_free(vzScene s)
+{
+    void* ptrs[] = {s->d_sph_in, s->d_mat_in, s->d_keys[0], s->d_keys[1], s->d_vals[0], s->d_vals[1],
+                    s->d_hist, s->d_bounds, s->d_sph, s->d_leaf_info, s->d_mat, s->d_nodes,
+                    s->d_boxes, s->d_child, s->d_parent_int, s->d_parent_leaf, s->d_visit,
+                    s->d_height};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+static vzStatus refit_core(vzScene s, hipStream_t st)
+{
+    uint32_t n = s->n;
+    if (n < 2) return VZ_SUCCESS;
+    hipLaunchKernelGGL(k_zero, dim3(blocks_for(n - 1, kBlk)), dim3(kBlk), 0, st, s->d_visit, n - 1);
+    hipLaunchKernelGGL(k_refit, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph, n, s->d_child,
+                       s->d_parent_int, s->d_parent_leaf, s->d_visit, s->d_boxes, s->d_height,
+                       s->d_nodes);
+    VZ_HIP_TRY(hipGetLastError());
+    uint32_t h = 0;
+    VZ_HIP_TRY(hipMemcpyAsync(&h, s->d_height, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VZ_HIP_TRY(hipStreamSynchronize(st));
+    s->depth = h + 1;
+    if (s->depth > kMaxDepth)
+        return set_error(VZ_ERROR_CAPACITY, "BVH depth " + std::to_string(s->depth) +
+                                                " exceeds the traversal stack (" +
+                                                std::to_string(kMaxDepth) + ")");
+    return VZ_SUCCESS;
+}
+
+vzStatus lbvh_build(vzScene s, hipStream_t st)
+{
+    uint32_t n = s->n;
+    uint32_t nb = std::min<uint32_t>(1024, blocks_for(n, kBlk));
+    hipEvent_t e0, e1;
+    VZ_HIP_TRY(hipEventCreate(&e0));
+    VZ_HIP_TRY(hipEventCreate(&e1));
+    VZ_HIP_TRY(hipEventRecord(e0, st));
+    float* partial = s->d_bounds;
+    float* fin = s->d_bounds + 6 * 1024;
+    hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(kBlk), 0, st, s->d_sph_in, n, partial);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, st, partial, nb, fin);
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in, n, fin,
+                       s->d_keys[0], s->d_vals[0]);
+    uint32_t tiles = s->hist_blocks;
+    for (int pass = 0; pass < 4; pass++) {
+        int src = pass & 1, dst = src ^ 1;
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src], n, pass * 8,
+                           tiles, s->d_hist);
+        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, s->d_hist, 256 * tiles);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src],
+                           s->d_vals[src], n, pass * 8, tiles, s->d_hist, s->d_keys[dst], s->d_vals[dst]);
+    }
+    s->d_morton = s->d_keys[0];  // four passes: result back in buffer 0
+    s->d_leaf_prim = s->d_vals[0];
+    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in,
+                       s->d_mat_in, s->d_leaf_prim, n, s->d_sph, s->d_leaf_info, s->d_mat);
+    if (n >= 2) {
+        hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, kBlk)), dim3(kBlk), 0, st, s->d_morton, n,
+                           s->d_child, s->d_parent_int, s->d_parent_leaf, s->d_visit);
+    }
+    VZ_HIP_TRY(hipGetLastError());
+    vzStatus rs = refit_core(s, st);
+    VZ_HIP_TRY(hipEventRecord(e1, st));
+    VZ_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    VZ_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    s->build_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (n < 2) s->depth = 1;
+    return rs;
+}
+
+vzStatus lbvh_refit(vzScene s, hipStream_t st)
+{
+    uint32_t n = s->n;
+    hipEvent_t e0, e1;
+    VZ_HIP_TRY(hipEventCreate(&e0));
+    VZ_HIP_TRY(hipEventCreate(&e1));
+    VZ_HIP_TRY(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_reorder_sph, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in,
+                       s->d_leaf_prim, n, s->d_sph);
+    VZ_HIP_TRY(hipGetLastError());
+    vzStatus rs = refit_core(s, st);
+    VZ_HIP_TRY(hipEventRecord(e1, st));
+    VZ_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    VZ_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    s->build_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rs;
+}
+
+}  // namespace vz

@@ -1,0 +1,161 @@
+// vz_traverse.h -- closest-hit traversal of the LBVH on gfx950, replacing
+// the traversal inside optixTrace (Core/RayTracer.cu:190-202) and the
+// __intersection__hit_sphere program it calls (Core/RayTracer.cu:229-270).
+//
+// Per lane: the ray in registers, a short stack of (node, tnear) in LDS laid
+// out [slot][lane] so a wave's pushes/pops hit distinct banks, and a private
+// overflow array that holds the oldest entries when the LDS ring is full.
+// Nodes are fetched as two 16-byte loads; lanes that visit the same node
+// coalesce into one cache-line access.  Near-first order, leaf children tested
+// as soon as their box is hit, popped entries culled against the current best
+// hit.  Ties resolve to the smaller t, then the smaller primitive id, so the
+// result is the exact lexicographic minimum over all spheres whatever the
+// visit order (the CPU oracle checks this against brute force).
+#pragma once
+#include "vz_device.h"
+#include "vz_internal.h"
+
+namespace vz {
+
+constexpr float kRobust = 1.0f + 0x1p-20f;  // >= 1 + 2*gamma_3 (Ize 2013)
+
+struct TraceResult {
+    float t;
+    uint32_t leaf;  // leaf (Morton-order) index, 0xFFFFFFFF on a miss
+};
+
+// S: LDS slots per lane (power of two).  lds_node/lds_t point at this lane's
+// column; consecutive slots are `stride` words apart.
+template <int S, bool COUNT>
+VZ_DEV TraceResult closest_hit(const SceneView& sc, v3 o, v3 d, float tmin, float tmax,
+                               uint32_t* lds_node, float* lds_t, uint32_t stride, uint32_t& nv,
+                               uint32_t& ns)
+{
+    float best = tmax;
+    uint32_t bleaf = 0xFFFFFFFFu, bprim = 0xFFFFFFFFu;
+    if (sc.n == 1u) {
+        float t;
+        if (COUNT) ns++;
+        if (hit_sphere(o, d, tmin, best, sc.spheres[0], t)) {
+            best = t;
+            bleaf = 0;
+        }
+        return TraceResult{best, bleaf};
+    }
+    float inv[3], org[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float dk = dd[k];
+        if (!(fabsf(dk) > 1e-20f)) dk = copysignf(1e-20f, dk);
+        inv[k] = 1.0f / dk;
+    }
+    uint32_t ov_node[kMaxDepth];
+    float ov_t[kMaxDepth];
+    uint32_t sp = 0, bottom = 0;
+    uint32_t node = 0;
+    // a well-formed tree visits each of its n-1 internal nodes at most once;
+    // the guards below only matter for a corrupted node array
+    uint32_t budget = sc.n;
+    for (;;) {
+        if (budget-- == 0u) break;
+        const uint4 A = sc.nodes[2 * node];
+        const uint4 B = sc.nodes[2 * node + 1];
+        if (COUNT) nv++;
+        const float no[3] = {__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z)};
+        const uint32_t meta = A.w;
+        const uint32_t qw[3] = {B.x, B.y, B.z};
+        const uint32_t g = B.w;
+        if (g + 1u >= sc.n) break;
+        float tn[2];
+        bool hit[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            float lo_t = tmin, hi_t = best;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                float s = scale_of(meta, k);
+                float lo = __builtin_fmaf(s, (float)((qw[k] >> (16 * c)) & 0xFFu), no[k]);
+                float hi = __builtin_fmaf(s, (float)((qw[k] >> (16 * c + 8)) & 0xFFu), no[k]);
+                float t0 = (lo - org[k]) * inv[k];
+                float t1 = (hi - org[k]) * inv[k];
+                lo_t = fmaxf(lo_t, fminf(t0, t1));
+                hi_t = fminf(hi_t, fmaxf(t0, t1));
+            }
+            tn[c] = lo_t;
+            hit[c] = lo_t <= hi_t * kRobust;
+        }
+        const uint32_t flags = meta >> 24;
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            if (hit[c] && ((flags >> c) & 1u)) {
+                uint32_t leaf = g + (uint32_t)c;
+                float t;
+                if (COUNT) ns++;
+                if (hit_sphere(o, d, tmin, best, sc.spheres[leaf], t)) {
+                    if (t < best) {
+                        best = t;
+                        bleaf = leaf;
+                        bprim = 0xFFFFFFFFu;  // looked up lazily on a tie
+                    } else {
+                        uint32_t pid = sc.leaf_info[leaf] & 0x3FFFFFFFu;
+                        if (bprim == 0xFFFFFFFFu && bleaf != 0xFFFFFFFFu)
+                            bprim = sc.leaf_info[bleaf] & 0x3FFFFFFFu;
+                        if (pid < bprim) {
+                            best = t;
+                            bleaf = leaf;
+                            bprim = pid;
+                        }
+                    }
+                }
+            }
+        }
+        const bool in0 = hit[0] && !(flags & 1u);
+        const bool in1 = hit[1] && !(flags & 2u);
+        if (in1 && g + 2u >= sc.n) break;  // internal child g+1 must be <= n-2
+        if (in0 && in1) {
+            const int nr = (tn[1] < tn[0]) ? 1 : 0;
+            const uint32_t far_node = g + (uint32_t)(1 - nr);
+            const float far_t = tn[1 - nr];
+            if (sp - bottom == (uint32_t)S) {  // ring full: oldest entry to the overflow
+                if (bottom >= kMaxDepth) break;
+                ov_node[bottom] = lds_node[(bottom & (S - 1)) * stride];
+                ov_t[bottom] = lds_t[(bottom & (S - 1)) * stride];
+                bottom++;
+            }
+            lds_node[(sp & (S - 1)) * stride] = far_node;
+            lds_t[(sp & (S - 1)) * stride] = far_t;
+            sp++;
+            node = g + (uint32_t)nr;
+            continue;
+        } else if (in0) {
+            node = g;
+            continue;
+        } else if (in1) {
+            node = g + 1u;
+            continue;
+        }
+        bool found = false;
+        while (sp > 0u) {
+            sp--;
+            uint32_t nd;
+            float t;
+            if (sp < bottom) {
+                nd = ov_node[sp];
+                t = ov_t[sp];
+                bottom = sp;
+            } else {
+                nd = lds_node[(sp & (S - 1)) * stride];
+                t = lds_t[(sp & (S - 1)) * stride];
+            }
+            if (t <= best * kRobust) {
+                node = nd;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return TraceResult{best, bleaf};
+}
+
+}  // namespace vz
