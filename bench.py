@@ -1,0 +1,169 @@
+"""Benchmark: Mray-segments/s of the plain method on the 1080p book scene.
+
+Workload (one step): the book final scene (486 spheres, mt19937 seed 5489),
+1920x1080, 256 samples per pixel, max depth 50, FP32, rendered by one
+vzLaunch (LBVH resident, wavefront integrator).  Inputs are resident in HBM
+before the timed region; the LBVH build is timed separately.
+
+Prints ONE JSON line.  Besides value/ms_per_step it carries:
+  roofline      the trace kernel (dominant): algorithmic bytes per launch /
+                its mean launch time from HIP events recorded around every
+                trace launch inside the timed steps, against 8 TB/s HBM
+  cpu_baseline  the CPU oracle (same integrator, FP32) on a row sample of the
+                same workload, on this host's cores
+  checks        segment count vs the oracle on the sample rows, RMSE vs the
+                oracle (FP32) and vs the FP64 oracle on a smaller row sample
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "Mray-segments/s @1080p; batched/served vs plain, p50/p99 latency, RMSE≤1e-3"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=256)
+    ap.add_argument("--depth", type=int, default=50)
+    ap.add_argument("--cpu-row-step", type=int, default=8, help="CPU baseline renders every k-th row")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is design only, see DESIGN.md)"}))
+        sys.exit(2)
+    if not (os.path.exists(os.path.join(ROOT, "venusaur_amd", "libvz.so"))
+            and os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so"))):
+        import __graft_entry__
+        __graft_entry__.build()
+    from venusaur_amd import api
+    from venusaur_amd.hipmem import DeviceBuffer, synchronize
+
+    W, H, SPP, DEPTH = a.width, a.height, a.spp, a.depth
+    aspect = float(np.float32(W) / np.float32(H))
+    ctx = api.Context(0)
+    spheres, mats = api.book_scene()
+    scene = api.Scene(ctx, spheres, mats)
+    build_ms = scene.info().build_ms
+    flags = api.MEGAKERNEL if a.mode == "megakernel" else 0
+    p = api.launch_params(W, H, SPP, DEPTH, [13, 2, 3], [0, 0, 0], 20.0, 0.1, 10.0, aspect=aspect)
+    acc = DeviceBuffer(W * H * 16)
+    img = DeviceBuffer(W * H * 4)
+    p.accum, p.image = acc.value, img.value
+
+    # untimed: warmup, then one counting pass for the algorithmic bytes
+    for _ in range(a.warmup):
+        scene.launch(p)
+    pc = api.LaunchParams.from_buffer_copy(p)
+    pc.flags = flags | api.COUNT_VISITS
+    cst = scene.launch(pc)
+
+    # timed region: K full renders; trace launches bracketed by HIP events
+    pt = api.LaunchParams.from_buffer_copy(p)
+    pt.flags = flags | api.TIME_STAGES
+    synchronize()
+    t0 = time.perf_counter()
+    stats = [scene.launch(pt) for _ in range(a.steps)]
+    synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    segs = sum(s.segments for s in stats)
+    value = segs / wall / 1e6
+    gpu_acc = acc.download(np.float32, (H, W, 4))
+
+    # roofline of the trace kernel.  Algorithmic bytes per segment: ray read
+    # (origin+dir 32 B, request word 4 B), 32 B per BVH node fetched, 16 B per
+    # sphere tested, the queue entry written (16 B shade entry on a hit, 4 B
+    # miss entry otherwise).
+    seg_c, nv_c, ns_c, hit_c = cst.segments, cst.node_visits, cst.sphere_tests, cst.hits
+    bytes_step = 36 * seg_c + 32 * nv_c + 16 * ns_c + 16 * hit_c + 4 * (seg_c - hit_c)
+    trace_ms = sum(s.trace_ms for s in stats)
+    launches = sum(s.trace_launches for s in stats)
+    roof = None
+    if a.mode == "wavefront" and trace_ms > 0:
+        per_launch_bytes = bytes_step * a.steps / launches
+        avg_ms = trace_ms / launches
+        achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "trace_kernel_traffic.json")
+        if os.path.exists(tf):
+            traffic = json.load(open(tf)).get("traffic_bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_wf_trace", "launches": launches, "avg_launch_ms": round(avg_ms, 4),
+                "trace_share_of_step": round(trace_ms / (sum(s.render_ms for s in stats)), 3),
+                "bytes_per_segment": round(bytes_step / seg_c, 1),
+                "node_visits_per_segment": round(nv_c / seg_c, 2),
+                "sphere_tests_per_segment": round(ns_c / seg_c, 2)}
+
+    # CPU baseline: the oracle's FP32 integrator on every k-th row
+    cpu = None
+    checks = {"gpu_segments_per_step": stats[0].segments, "paths_per_step": stats[0].paths,
+              "lbvh_build_ms": round(build_ms, 3)}
+    if not a.no_cpu:
+        import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        threads = min(threads, os.cpu_count() or threads)
+        cam = O.camera_lookat([13, 2, 3], [0, 0, 0], 20.0, aspect, 0.1, 10.0)
+        op = O.make_params(W, H, SPP, DEPTH, cam, row_step=a.cpu_row_step, threads=threads)
+        ob = O.BVH(spheres)
+        cacc, _, cst_cpu = O.render(spheres, mats, op, bvh=ob, want_image=False)
+        rows = np.arange(0, H, a.cpu_row_step)
+        cpu_rate = cst_cpu.segments / cst_cpu.seconds / 1e6
+        cpu = {"value": round(cpu_rate, 3), "unit": "Mray-segments/s", "cores": cst_cpu.threads,
+               "kind": "port",
+               "sample": f"rows y%{a.cpu_row_step}==0 ({len(rows)} of {H} rows, full {SPP} spp, depth {DEPTH}): "
+                         f"{cst_cpu.segments} segments in {cst_cpu.seconds:.1f} s",
+               "gpu_over_cpu": round(value / cpu_rate, 1)}
+        checks["rmse_vs_cpu_fp32_sample_rows"] = float(np.sqrt(np.mean(
+            (gpu_acc[rows, :, :3].astype(np.float64) - cacc[rows, :, :3]) ** 2)))
+        checks["bit_exact_sample_rows"] = bool(np.array_equal(gpu_acc[rows], cacc[rows]))
+        # FP64 oracle on a thinner sample
+        r64 = 64
+        op64 = O.make_params(W, H, SPP, DEPTH, cam, row_step=r64, threads=threads)
+        acc64, _ = O.render_f64(spheres, mats, op64, bvh=ob)
+        rows64 = np.arange(0, H, r64)
+        checks["rmse_vs_fp64_oracle"] = float(np.sqrt(np.mean(
+            (gpu_acc[rows64, :, :3].astype(np.float64) - acc64[rows64, :, :3]) ** 2)))
+        checks["rmse_cpu_fp32_vs_fp64"] = float(np.sqrt(np.mean(
+            (cacc[rows64, :, :3].astype(np.float64) - acc64[rows64, :, :3]) ** 2)))
+        checks["fp64_sample"] = f"rows y%{r64}==0"
+
+    info = ctx.info()
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mray-segments/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (book scene from mt19937 seed 5489)",
+        "config": {"workload": f"book scene {W}x{H} {SPP}spp depth {DEPTH} plain ({a.mode})",
+                   "spheres": int(len(spheres)), "width": W, "height": H, "spp": SPP,
+                   "max_depth": DEPTH, "device": info["name"], "cus": info["num_cus"]},
+        "roofline": roof, "cpu_baseline": cpu, "checks": checks,
+    }
+    print(json.dumps(out), flush=True)
+    scene.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -87,8 +87,9 @@ typedef struct vzStats {
     uint64_t sphere_tests;   /* leaf sphere tests (VZ_COUNT_VISITS only) */
     double render_ms;        /* wall time of the launch (HIP events) */
     double trace_ms;         /* sum of trace-kernel times (VZ_TIME_STAGES only) */
-    uint32_t trace_launches; /* trace-kernel launches that had work */
+    uint32_t trace_launches; /* trace-kernel launches (timed with VZ_TIME_STAGES) */
     uint32_t iterations;     /* wavefront iterations */
+    uint64_t hits;           /* segments that hit a sphere (the rest missed) */
 } vzStats;
 
 typedef struct vzAccelInfo {

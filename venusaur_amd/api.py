@@ -49,7 +49,7 @@ class LaunchParams(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("segments", C.c_uint64), ("paths", C.c_uint64), ("node_visits", C.c_uint64),
                 ("sphere_tests", C.c_uint64), ("render_ms", C.c_double), ("trace_ms", C.c_double),
-                ("trace_launches", C.c_uint32), ("iterations", C.c_uint32)]
+                ("trace_launches", C.c_uint32), ("iterations", C.c_uint32), ("hits", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

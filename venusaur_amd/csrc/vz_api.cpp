@@ -319,6 +319,7 @@ vzStatus vzLaunch(vzScene scene, const vzLaunchParams* params, vzStream stream, 
         stats->paths = rr.paths;
         stats->node_visits = rr.node_visits;
         stats->sphere_tests = rr.sphere_tests;
+        stats->hits = rr.hits;
         stats->render_ms = rr.render_ms;
         stats->trace_ms = rr.trace_ms;
         stats->trace_launches = rr.trace_launches;
@@ -401,6 +402,7 @@ vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n, cons
         stats->paths = rr.paths;
         stats->node_visits = rr.node_visits;
         stats->sphere_tests = rr.sphere_tests;
+        stats->hits = rr.hits;
         stats->render_ms = rr.render_ms;
         stats->trace_ms = rr.trace_ms;
         stats->trace_launches = rr.trace_launches;

@@ -1,24 +1,26 @@
 // vz_engine.h -- the wavefront integrator shared by the plain launch
 // (vzLaunch), the batched form (vzBatchLaunch) and the serving loop.
 //
-// One engine owns the ray queues (two parities), the shade and miss queues,
-// a device table of up to kMaxRequests requests and a scene table.  Each
-// wavefront iteration runs six kernels on one stream:
-//   trace   persistent; pulls 64-ray chunks of the current ray queue, finds
-//           the closest hit, compacts hits into the shade queue and misses
-//           into the miss queue (wave ballot + mbcnt + one atomic per wave)
-//   miss    radiance = throughput * sky, path ends
-//   shade   material scatter; survivors are compacted into the next queue
+// Layout.  The ray queue (two parities) is split into NB block-private
+// slices of R rays, one per persistent workgroup of the step kernel.  A
+// workgroup only ever touches its own slice, so claiming work and compacting
+// survivors use LDS atomics and wave ballots, never a device-wide atomic
+// (on MI355X a device-scope atomic to one address serialises across the 8
+// XCDs).  Each wavefront iteration runs four kernels on one stream:
+//   step    persistent: waves pull 64-ray chunks of their slice, trace them
+//           (LBVH traversal), run the miss stage (throughput * sky) or the
+//           shade stage (material scatter) in registers, and compact the
+//           surviving paths into the slice of the next parity
 //   control one workgroup: finds passes whose paths all ended, plans their
-//           accumulation, grants queue space to requests for new camera rays
+//           accumulation, grants free queue space to requests and splits
+//           the granted camera rays over the slices' free room
 //   apply   adds finished passes into per-pixel sums in sample order and
 //           writes accum/image when a request's last pass lands
-//   regen   camera rays for the granted items into the next queue
+//   regen   each workgroup writes the camera rays granted to its slice
 // A "pass" is sample s of every pixel of one request; its radiances land in
 // one of K ring slots, so per-pixel sums are formed in sample order whatever
 // order the paths finish in.  Iterations carry no host round trip; the host
-// only polls a status word between blocks of iterations (or hipGraphs of
-// them).
+// polls a status copy between blocks of iterations (or hipGraphs of them).
 #pragma once
 #include <vector>
 
@@ -26,6 +28,9 @@
 #include "vz_internal.h"
 
 namespace vz {
+
+constexpr uint32_t kMaxBlocks = 4096;   // step-kernel workgroups (queue slices)
+constexpr uint32_t kDeadRay = 0xFFFFFFFFu;  // meta of a padding item: skipped
 
 struct ReqDesc {             // device table entry, written by the host
     Cam cam;
@@ -39,7 +44,8 @@ struct ReqDesc {             // device table entry, written by the host
     float* sum32;            // 3 * npix
     double* sum64;           // 3 * npix (FP64 accumulation)
     uint32_t* camseed;       // npix
-    uint32_t* done;          // K pass counters
+    uint32_t* done;          // K pass counters (paths ended)
+    unsigned long long* segsum;  // K pass segment sums
     void* accum;
     uint32_t* image;
 };
@@ -50,18 +56,18 @@ struct ReqState {            // device, advanced by the control kernel
     uint32_t active;
     uint32_t finish_iter;
     uint32_t pad;
-    unsigned long long segments;
+    unsigned long long segments;  // segments of the passes accumulated so far
 };
 
-struct Counters {
-    uint32_t ray_count[2];
-    uint32_t trace_head;
-    uint32_t shade_count;
-    uint32_t miss_count;
+struct Counters {            // written by the control kernel only
     uint32_t iteration;
-    uint32_t active;          // requests still running
-    uint32_t n_slots;         // request slots in use (table prefix)
-    unsigned long long segments, paths, node_visits, sphere_tests;
+    uint32_t active;         // requests still running
+    uint32_t n_slots;        // request slots in use (table prefix)
+    uint32_t total_regen;
+};
+
+struct BlockStats {          // one per step/regen workgroup, single writer
+    unsigned long long hits, node_visits, sphere_tests, paths, segments;
 };
 
 struct Plan {
@@ -73,6 +79,8 @@ struct Plan {
     uint32_t regen_req[kMaxRequests];
     uint32_t regen_prefix[kMaxRequests];
     unsigned long long regen_base[kMaxRequests];
+    uint32_t blk_first[kMaxBlocks];   // first granted item of each slice
+    uint32_t blk_count[kMaxBlocks];   // granted items for each slice
 };
 
 struct EngineConfig {
@@ -89,23 +97,25 @@ struct DeviceBlock {
 struct Engine {
     vzContext ctx = nullptr;
     EngineConfig cfg;
-    uint32_t Q = 0;
+    uint32_t Q = 0;                        // rays in flight (NB * R)
+    uint32_t NB = 0, R = 0;                // slices and slice capacity
     float4* q_o[2] = {nullptr, nullptr};   // origin xyz, pixel
     float4* q_d[2] = {nullptr, nullptr};   // dir xyz, seed
     float4* q_t[2] = {nullptr, nullptr};   // throughput xyz, req<<16 | slot<<8 | depth
-    uint4* shade_q = nullptr;              // ray index, t, leaf
-    uint32_t* miss_q = nullptr;
+    uint32_t* d_bcnt = nullptr;            // 2 * NB slice counts
+    BlockStats* d_bstats = nullptr;        // NB partial stats
     ReqDesc* d_req = nullptr;
     ReqState* d_state = nullptr;
     SceneView* d_scenes = nullptr;
     uint32_t scene_cap = 0;
     Counters* d_cnt = nullptr;
     Plan* d_plan = nullptr;
-    Counters* h_cnt = nullptr;             // pinned copy for polling
+    Counters* h_cnt = nullptr;             // pinned copies for polling (2)
     ReqState* h_state = nullptr;           // pinned copy of the request states
     ReqDesc* h_req = nullptr;              // pinned staging for the table
     SceneView* h_scenes = nullptr;
-    uint32_t grid_trace = 0, grid_small = 0;
+    BlockStats* h_bstats = nullptr;
+    uint32_t grid_small = 0;
     hipGraphExec_t graph = nullptr;
     hipStream_t graph_stream = nullptr;
     bool graph_count = false;
@@ -140,7 +150,7 @@ struct RunOptions {
 };
 
 struct RunResult {
-    unsigned long long segments = 0, paths = 0, node_visits = 0, sphere_tests = 0;
+    unsigned long long segments = 0, paths = 0, node_visits = 0, sphere_tests = 0, hits = 0;
     double render_ms = 0.0, trace_ms = 0.0;
     uint32_t trace_launches = 0, iterations = 0;
     std::vector<unsigned long long> req_segments;

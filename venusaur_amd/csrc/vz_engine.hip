@@ -1,7 +1,8 @@
-// vz_engine.hip -- wavefront integrator (see vz_engine.h for the stage
-// list).  Replaces the recursive optixTrace chain of the reference's OptiX
-// programs (Core/RayTracer.cu:163-450) with queue-driven persistent kernels,
-// after Laine, Karras and Aila, "Megakernels Considered Harmful" (HPG 2013).
+// vz_engine.hip -- wavefront integrator (see vz_engine.h for the layout and
+// the stage list).  Replaces the recursive optixTrace chain of the
+// reference's OptiX programs (Core/RayTracer.cu:163-450) with queue-driven
+// persistent kernels, after Laine, Karras and Aila, "Megakernels Considered
+// Harmful" (HPG 2013).
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -12,48 +13,42 @@
 namespace vz {
 namespace {
 
+constexpr int kStepBlk = 512;   // step-kernel workgroup (8 waves)
+constexpr int kStackS = 8;      // LDS stack slots per lane
+constexpr int kTbl = 128;       // LDS (request, pass) tally table entries
+constexpr int kRegenBlk = 256;
 constexpr int kBlk = 256;
-constexpr int kStackS = 8;  // LDS stack slots per lane in the trace kernel
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 __device__ __forceinline__ uint32_t bcast(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
 
-// Appends the lanes with `pred` to a queue: one atomic per wave, slots in
-// lane order.  Returns the lane's slot (undefined when !pred).
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+// Adds (count, segments) for `key` to the workgroup's LDS tally.  Returns
+// false when the table is full (the caller then goes to global memory).
+__device__ __forceinline__ bool tally_add(uint32_t* keys, uint32_t* cnt, unsigned long long* seg,
+                                          uint32_t key, uint32_t c, unsigned long long s)
 {
-    uint64_t m = __ballot(pred);
-    uint32_t n = (uint32_t)__popcll(m);
-    uint32_t base = 0;
-    if (n) {
-        if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1)) base = atomicAdd(counter, n);
-        base = __shfl(base, __ffsll((unsigned long long)m) - 1);
+    uint32_t h = (key * 2654435761u) >> 25;  // 7 bits: kTbl == 128
+    for (int probe = 0; probe < kTbl; probe++) {
+        uint32_t prev = atomicCAS(&keys[h], kEmpty, key);
+        if (prev == kEmpty || prev == key) {
+            atomicAdd(&cnt[h], c);
+            atomicAdd(&seg[h], s);
+            return true;
+        }
+        h = (h + 1u) & (kTbl - 1);
     }
-    return base + mbcnt(m);
-}
-
-// Adds 1 per lane with `pred` to counter[key]: lanes sharing a key are
-// summed first so each distinct key costs one atomic per wave.
-template <typename F>
-__device__ __forceinline__ void wave_count_by_key(bool pred, uint32_t key, F&& add)
-{
-    uint64_t todo = __ballot(pred);
-    while (todo) {
-        int leader = __ffsll((unsigned long long)todo) - 1;
-        uint32_t lk = __shfl(key, leader);
-        uint64_t same = __ballot(pred && key == lk) & todo;
-        if ((int)__lane_id() == leader) add(lk, (uint32_t)__popcll(same));
-        todo &= ~same;
-    }
+    return false;
 }
 
 __device__ __forceinline__ void terminate(const ReqDesc& D, uint32_t slot, uint32_t pixel, v3 L)
@@ -64,164 +59,161 @@ __device__ __forceinline__ void terminate(const ReqDesc& D, uint32_t slot, uint3
     pb[2u * D.npix + pixel] = L.z;
 }
 
-// ---- trace ------------------------------------------------------------------
+// ---- step: trace + miss/shade stages + compaction, one slice per workgroup ----
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlk) k_wf_trace(const SceneView* __restrict__ scenes,
-                                                   const ReqDesc* __restrict__ req,
-                                                   ReqState* __restrict__ st,
-                                                   const float4* __restrict__ q_o,
-                                                   const float4* __restrict__ q_d,
-                                                   const float4* __restrict__ q_t, Counters* cnt,
-                                                   int p, uint4* __restrict__ shade_q,
-                                                   uint32_t* __restrict__ miss_q)
+__global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restrict__ scenes,
+                                                      const ReqDesc* __restrict__ req,
+                                                      const float4* __restrict__ q_o,
+                                                      const float4* __restrict__ q_d,
+                                                      const float4* __restrict__ q_t,
+                                                      float4* __restrict__ n_o, float4* __restrict__ n_d,
+                                                      float4* __restrict__ n_t,
+                                                      const uint32_t* __restrict__ cnt_cur,
+                                                      uint32_t* __restrict__ cnt_next,
+                                                      BlockStats* __restrict__ bstats, uint32_t R)
 {
-    __shared__ uint32_t s_node[kStackS * kBlk];
-    __shared__ float s_t[kStackS * kBlk];
-    const uint32_t count = cnt->ray_count[p];
-    const uint32_t lane = __lane_id();
-    uint32_t seg = 0, nv = 0, ns = 0;
+    __shared__ uint32_t s_node[kStackS * kStepBlk];
+    __shared__ float s_t[kStackS * kStepBlk];
+    __shared__ uint32_t t_key[kTbl], t_cnt[kTbl];
+    __shared__ unsigned long long t_seg[kTbl];
+    __shared__ uint32_t s_head, s_out;
+    __shared__ unsigned long long s_red[4][kStepBlk / 64];
+    const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+    for (uint32_t i = tid; i < (uint32_t)kTbl; i += kStepBlk) {
+        t_key[i] = kEmpty;
+        t_cnt[i] = 0u;
+        t_seg[i] = 0ull;
+    }
+    if (tid == 0) {
+        s_head = 0u;
+        s_out = 0u;
+    }
+    __syncthreads();
+    const uint32_t b = blockIdx.x;
+    const size_t base_off = (size_t)b * R;
+    const uint32_t count = cnt_cur[b];
+    uint32_t nv = 0, ns = 0, nh = 0, seg = 0;
     for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&cnt->trace_head, 64u);
-        base = bcast(base);
-        if (base >= count) break;
-        uint32_t i = base + lane;
-        bool valid = i < count;
-        TraceResult r{0.0f, 0xFFFFFFFFu};
-        uint32_t rq = 0;
+        uint32_t c0 = 0;
+        if (lane == 0) c0 = atomicAdd(&s_head, 64u);
+        c0 = bcast(c0);
+        if (c0 >= count) break;
+        const uint32_t local = c0 + lane;
+        bool valid = local < count;
+        float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4, t4 = o4;
+        uint32_t meta = kDeadRay;
         if (valid) {
-            float4 o = q_o[i], d = q_d[i];
-            rq = __float_as_uint(q_t[i].w) >> 16;
-            const SceneView sc = scenes[req[rq].scene];
-            r = closest_hit<kStackS, COUNT>(sc, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e16f,
-                                            s_node + threadIdx.x, s_t + threadIdx.x, kBlk, nv, ns);
-            seg++;
+            o4 = q_o[base_off + local];
+            d4 = q_d[base_off + local];
+            t4 = q_t[base_off + local];
+            meta = __float_as_uint(t4.w);
+            valid = meta != kDeadRay;
         }
-        bool hit = valid && r.leaf != 0xFFFFFFFFu;
-        bool miss = valid && !hit;
-        uint32_t hs = wave_append(hit, &cnt->shade_count);
-        if (hit) shade_q[hs] = make_uint4(i, __float_as_uint(r.t), r.leaf, 0u);
-        uint32_t ms = wave_append(miss, &cnt->miss_count);
-        if (miss) miss_q[ms] = i;
-        wave_count_by_key(valid, rq, [&](uint32_t k, uint32_t c) {
-            atomicAdd(&st[k].segments, (unsigned long long)c);
-        });
-    }
-    seg = wave_sum(seg);
-    if (COUNT) {
-        nv = wave_sum(nv);
-        ns = wave_sum(ns);
-    }
-    if (lane == 0 && seg) {
-        atomicAdd(&cnt->segments, (unsigned long long)seg);
-        if (COUNT) {
-            atomicAdd(&cnt->node_visits, (unsigned long long)nv);
-            atomicAdd(&cnt->sphere_tests, (unsigned long long)ns);
-        }
-    }
-}
-
-// ---- miss -------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlk) k_wf_miss(const ReqDesc* __restrict__ req,
-                                                  const float4* __restrict__ q_o,
-                                                  const float4* __restrict__ q_d,
-                                                  const float4* __restrict__ q_t,
-                                                  const Counters* __restrict__ cnt,
-                                                  const uint32_t* __restrict__ miss_q)
-{
-    const uint32_t count = cnt->miss_count;
-    const uint32_t lane = __lane_id();
-    const uint32_t wave = (blockIdx.x * kBlk + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * kBlk) >> 6;
-    for (uint32_t base = wave * 64u; base < count; base += nwaves * 64u) {
-        uint32_t j = base + lane;
-        bool valid = j < count;
-        uint32_t key = 0;
-        const ReqDesc* D = nullptr;
-        if (valid) {
-            uint32_t i = miss_q[j];
-            float4 d = q_d[i], t = q_t[i];
-            uint32_t meta = __float_as_uint(t.w);
-            uint32_t rq = meta >> 16, slot = (meta >> 8) & 0xFFu;
-            D = &req[rq];
-            v3 L = mul(mk(t.x, t.y, t.z), sky(mk(d.x, d.y, d.z)));
-            terminate(*D, slot, __float_as_uint(q_o[i].w), L);
-            key = (rq << 8) | slot;
-        }
-        wave_count_by_key(valid, key, [&](uint32_t k, uint32_t c) {
-            atomicAdd(&req[k >> 8].done[k & 0xFFu], c);
-        });
-    }
-}
-
-// ---- shade --------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlk) k_wf_shade(const SceneView* __restrict__ scenes,
-                                                   const ReqDesc* __restrict__ req,
-                                                   const float4* __restrict__ q_o,
-                                                   const float4* __restrict__ q_d,
-                                                   const float4* __restrict__ q_t, Counters* cnt,
-                                                   int p, const uint4* __restrict__ shade_q,
-                                                   float4* __restrict__ n_o, float4* __restrict__ n_d,
-                                                   float4* __restrict__ n_t)
-{
-    const uint32_t count = cnt->shade_count;
-    const uint32_t lane = __lane_id();
-    const uint32_t wave = (blockIdx.x * kBlk + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * kBlk) >> 6;
-    uint32_t* next_count = &cnt->ray_count[1 - p];
-    for (uint32_t base = wave * 64u; base < count; base += nwaves * 64u) {
-        uint32_t j = base + lane;
-        bool valid = j < count;
-        bool survive = false, ended = false;
-        uint32_t key = 0, meta = 0, seed = 0, pixel = 0;
+        bool ended = false, survive = false;
+        uint32_t key = 0, pathseg = 0, seed = 0;
         v3 pos = mk(0.0f, 0.0f, 0.0f), nd = pos, thr = pos;
         if (valid) {
-            uint4 e = shade_q[j];
-            uint32_t i = e.x;
-            float t = __uint_as_float(e.y);
-            uint32_t leaf = e.z;
-            float4 o4 = q_o[i], d4 = q_d[i], t4 = q_t[i];
-            meta = __float_as_uint(t4.w);
-            uint32_t rq = meta >> 16, slot = (meta >> 8) & 0xFFu, depth = meta & 0xFFu;
-            pixel = __float_as_uint(o4.w);
-            seed = __float_as_uint(d4.w);
+            const uint32_t rq = meta >> 16, slot = (meta >> 8) & 0xFFu, depth = meta & 0xFFu;
             const ReqDesc* D = &req[rq];
-            key = (rq << 8) | slot;
-            ended = true;
+            const SceneView sc = scenes[D->scene];
+            const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+            const TraceResult r = closest_hit<kStackS, COUNT>(sc, o, d, 0.001f, 1e16f, s_node + tid, s_t + tid,
+                                                              kStepBlk, nv, ns);
+            seg++;
+            seed = __float_as_uint(d4.w);
             v3 L = mk(0.0f, 0.0f, 0.0f);
-            if (depth > 0u) {
-                const SceneView sc = scenes[D->scene];
-                float4 s = sc.spheres[leaf];
-                uint32_t type = sc.leaf_info[leaf] >> 30;
-                float4 mat = sc.mats[leaf];
-                v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
-                pos = add(o, scl(d, t));
-                v3 n = scl(sub(pos, mk(s.x, s.y, s.z)), 1.0f / s.w);
-                bool front = dot(d, n) < 0.0f;
-                if (!front) n = neg(n);
-                v3 att;
-                if (scatter(type, mat, d, n, front, seed, nd, att)) {
-                    thr = mul(mk(t4.x, t4.y, t4.z), att);
-                    meta = meta - 1u;
-                    survive = true;
-                    ended = false;
+            ended = true;
+            if (r.leaf == 0xFFFFFFFFu) {
+                // miss stage (RayTracer.cu:442-450)
+                L = mul(mk(t4.x, t4.y, t4.z), sky(d));
+            } else {
+                nh++;
+                if (depth > 0u) {
+                    // shade stage (RayTracer.cu:272-440)
+                    const float4 s = sc.spheres[r.leaf];
+                    const uint32_t type = sc.leaf_info[r.leaf] >> 30;
+                    const float4 mat = sc.mats[r.leaf];
+                    pos = add(o, scl(d, r.t));
+                    v3 n = scl(sub(pos, mk(s.x, s.y, s.z)), 1.0f / s.w);
+                    const bool front = dot(d, n) < 0.0f;
+                    if (!front) n = neg(n);
+                    v3 att;
+                    if (scatter(type, mat, d, n, front, seed, nd, att)) {
+                        thr = mul(mk(t4.x, t4.y, t4.z), att);
+                        meta = meta - 1u;
+                        survive = true;
+                        ended = false;
+                    }
                 }
             }
-            if (ended) terminate(*D, slot, pixel, L);
+            if (ended) {
+                terminate(*D, slot, __float_as_uint(o4.w), L);
+                key = (rq << 8) | slot;
+                pathseg = D->max_depth - depth;
+            }
         }
-        uint32_t k = wave_append(survive, next_count);
-        if (survive) {
-            n_o[k] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(pixel));
-            n_d[k] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(seed));
-            n_t[k] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(meta));
+        // survivors: one LDS atomic per wave, slots in lane order
+        const uint64_t sm = __ballot(survive);
+        if (sm) {
+            uint32_t p0 = 0;
+            if (lane == (uint32_t)(__ffsll((unsigned long long)sm) - 1)) p0 = atomicAdd(&s_out, (uint32_t)__popcll(sm));
+            p0 = __shfl(p0, __ffsll((unsigned long long)sm) - 1);
+            if (survive) {
+                const size_t k = base_off + p0 + mbcnt(sm);
+                n_o[k] = make_float4(pos.x, pos.y, pos.z, o4.w);
+                n_d[k] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(seed));
+                n_t[k] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(meta));
+            }
         }
-        wave_count_by_key(ended, key, [&](uint32_t kk, uint32_t c) {
-            atomicAdd(&req[kk >> 8].done[kk & 0xFFu], c);
-        });
+        // ended paths: per distinct (request, pass) one tally update per wave
+        uint64_t todo = __ballot(ended);
+        while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const uint32_t lk = __shfl(key, leader);
+            const bool mine = ended && key == lk;
+            const uint64_t same = __ballot(mine) & todo;
+            const uint32_t sg = wave_sum<uint32_t>(mine ? pathseg : 0u);
+            if ((int)lane == leader) {
+                const uint32_t c = (uint32_t)__popcll(same);
+                if (!tally_add(t_key, t_cnt, t_seg, lk, c, sg)) {
+                    atomicAdd(&req[lk >> 8].done[lk & 0xFFu], c);
+                    atomicAdd(&req[lk >> 8].segsum[lk & 0xFFu], (unsigned long long)sg);
+                }
+            }
+            todo &= ~same;
+        }
+    }
+    // workgroup epilogue: flush the tally, publish the slice count and stats
+    const uint32_t w_seg = wave_sum(seg), w_nh = wave_sum(nh);
+    const uint32_t w_nv = COUNT ? wave_sum(nv) : 0u, w_ns = COUNT ? wave_sum(ns) : 0u;
+    if (lane == 0) {
+        s_red[0][wave] = w_seg;
+        s_red[1][wave] = w_nh;
+        s_red[2][wave] = w_nv;
+        s_red[3][wave] = w_ns;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < (uint32_t)kTbl; i += kStepBlk) {
+        const uint32_t k = t_key[i];
+        if (k != kEmpty) {
+            atomicAdd(&req[k >> 8].done[k & 0xFFu], t_cnt[i]);
+            atomicAdd(&req[k >> 8].segsum[k & 0xFFu], t_seg[i]);
+        }
+    }
+    if (tid == 0) {
+        cnt_next[b] = s_out;
+        unsigned long long a[4] = {0, 0, 0, 0};
+        for (int w = 0; w < kStepBlk / 64; w++)
+            for (int q = 0; q < 4; q++) a[q] += s_red[q][w];
+        BlockStats& B = bstats[b];
+        B.segments += a[0];
+        B.hits += a[1];
+        B.node_visits += a[2];
+        B.sphere_tests += a[3];
     }
 }
 
-// ---- control (one workgroup of 1024 threads, one per request slot) -------------
+// ---- control (one workgroup of 1024 threads) -----------------------------------
 __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s, uint32_t* total)
 {
     const uint32_t t = threadIdx.x;
@@ -239,14 +231,29 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s, uint32_t* total)
     return incl - v;
 }
 
+constexpr uint32_t kSlicesPerThread = kMaxBlocks / 1024;
+
 __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__ req,
                                                      ReqState* __restrict__ st, Counters* cnt,
-                                                     Plan* __restrict__ plan, uint32_t Q, int p)
+                                                     Plan* __restrict__ plan,
+                                                     const uint32_t* __restrict__ cnt_next, uint32_t NB,
+                                                     uint32_t R)
 {
     __shared__ uint32_t s_scan[1024];
     const uint32_t r = threadIdx.x;
     const uint32_t nslots = cnt->n_slots;
     const uint32_t iter = cnt->iteration;
+    // 1. free room of the slices this thread owns (slices r, r+1024, ...)
+    uint32_t fr[kSlicesPerThread];
+    uint32_t my_free = 0;
+    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
+        uint32_t sb = r + q * 1024u;
+        fr[q] = sb < NB ? R - cnt_next[sb] : 0u;
+        my_free += fr[q];
+    }
+    uint32_t freeq;
+    const uint32_t free_pre = block_excl_scan(my_free, s_scan, &freeq);
+    // 2. per request: passes whose paths all ended, and the camera-ray quota
     uint32_t acc_first = 0, acc_cnt = 0, quota = 0, npix = 0;
     bool running = false, still = false;
     unsigned long long next_item = 0;
@@ -258,7 +265,10 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         uint32_t a = S.acc_next, c = 0;
         while (a + c < D.spp && (unsigned long long)(a + c + 1) * D.npad <= S.next_item &&
                D.done[(a + c) % D.K] == D.npix) {
-            D.done[(a + c) % D.K] = 0u;
+            const uint32_t slot = (a + c) % D.K;
+            S.segments += D.segsum[slot];
+            D.done[slot] = 0u;
+            D.segsum[slot] = 0ull;
             c++;
         }
         acc_first = a;
@@ -278,19 +288,17 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         st[r].acc_next = S.acc_next;
         st[r].active = S.active;
         st[r].finish_iter = S.finish_iter;
+        st[r].segments = S.segments;
     }
-    const uint32_t live = cnt->ray_count[1 - p];
-    const uint32_t freeq = Q > live ? Q - live : 0u;
     uint32_t tq;
-    uint32_t excl = block_excl_scan(quota, s_scan, &tq);
-    uint32_t grant = excl >= freeq ? 0u : min(quota, freeq - excl);
+    const uint32_t excl = block_excl_scan(quota, s_scan, &tq);
+    const uint32_t grant = excl >= freeq ? 0u : min(quota, freeq - excl);
     if (running && grant) st[r].next_item = next_item + grant;
-    // compact the accumulation plan
     uint32_t nA, totA, nR, totR, nact;
-    uint32_t posA = block_excl_scan(acc_cnt ? 1u : 0u, s_scan, &nA);
-    uint32_t pixA = block_excl_scan(acc_cnt ? npix : 0u, s_scan, &totA);
-    uint32_t posR = block_excl_scan(grant ? 1u : 0u, s_scan, &nR);
-    uint32_t preR = block_excl_scan(grant, s_scan, &totR);
+    const uint32_t posA = block_excl_scan(acc_cnt ? 1u : 0u, s_scan, &nA);
+    const uint32_t pixA = block_excl_scan(acc_cnt ? npix : 0u, s_scan, &totA);
+    const uint32_t posR = block_excl_scan(grant ? 1u : 0u, s_scan, &nR);
+    const uint32_t preR = block_excl_scan(grant, s_scan, &totR);
     (void)block_excl_scan(still ? 1u : 0u, s_scan, &nact);
     if (acc_cnt) {
         plan->acc_req[posA] = r;
@@ -303,16 +311,23 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         plan->regen_prefix[posR] = preR;
         plan->regen_base[posR] = next_item;
     }
+    // 3. split the granted camera rays over the slices, in slice order
+    uint32_t run = free_pre;
+    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
+        uint32_t sb = r + q * 1024u;
+        if (sb < NB) {
+            plan->blk_first[sb] = run;
+            plan->blk_count[sb] = totR > run ? min(fr[q], totR - run) : 0u;
+        }
+        run += fr[q];
+    }
     if (r == 0) {
         plan->n_acc = nA;
         plan->total_acc_pix = totA;
         plan->n_regen = nR;
         plan->total_regen = totR;
         cnt->active = nact;
-        cnt->ray_count[p] = 0u;
-        cnt->trace_head = 0u;
-        cnt->shade_count = 0u;
-        cnt->miss_count = 0u;
+        cnt->total_regen = totR;
         cnt->iteration = iter + 1u;
     }
 }
@@ -389,53 +404,56 @@ __global__ void __launch_bounds__(kBlk) k_wf_apply(const ReqDesc* __restrict__ r
     }
 }
 
-// ---- regen: camera rays for the granted items ------------------------------------
-__global__ void __launch_bounds__(kBlk) k_wf_regen(const ReqDesc* __restrict__ req,
-                                                   const Plan* __restrict__ plan, Counters* cnt,
-                                                   int p, float4* __restrict__ n_o,
-                                                   float4* __restrict__ n_d, float4* __restrict__ n_t)
+// ---- regen: camera rays for the items granted to each slice ------------------------
+__global__ void __launch_bounds__(kRegenBlk) k_wf_regen(const ReqDesc* __restrict__ req,
+                                                        const Plan* __restrict__ plan,
+                                                        uint32_t* __restrict__ cnt_next,
+                                                        float4* __restrict__ n_o, float4* __restrict__ n_d,
+                                                        float4* __restrict__ n_t,
+                                                        BlockStats* __restrict__ bstats, uint32_t R)
 {
-    const uint32_t total = plan->total_regen, n = plan->n_regen;
-    const uint32_t lane = __lane_id();
-    const uint32_t wave = (blockIdx.x * kBlk + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * kBlk) >> 6;
-    uint32_t* next_count = &cnt->ray_count[1 - p];
+    __shared__ uint32_t s_paths[kRegenBlk / 64];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    const uint32_t cnt = plan->blk_count[b], first = plan->blk_first[b], nreg = plan->n_regen;
+    const uint32_t pos0 = cnt_next[b];
+    const size_t base_off = (size_t)b * R + pos0;
     uint32_t paths = 0;
-    for (uint32_t base = wave * 64u; base < total; base += nwaves * 64u) {
-        uint32_t j = base + lane;
-        bool valid = false;
-        v3 o = mk(0.0f, 0.0f, 0.0f), d = o;
-        uint32_t pixel = 0, seed = 0, meta = 0;
-        if (j < total) {
-            uint32_t e = find_entry(plan->regen_prefix, n, j);
-            uint32_t rq = plan->regen_req[e];
-            const ReqDesc& D = req[rq];
-            unsigned long long item = plan->regen_base[e] + (j - plan->regen_prefix[e]);
-            uint32_t s = (uint32_t)(item / D.npad);
-            uint32_t i = (uint32_t)(item % D.npad);
-            uint32_t tsz = D.tw * D.th;
-            uint32_t tile = i / tsz, w = i % tsz;
-            uint32_t x = (tile % D.tiles_x) * D.tw + w % D.tw;
-            uint32_t y = (tile / D.tiles_x) * D.th + w / D.tw;
-            if (x < D.cam.width && y < D.cam.height) {
-                valid = true;
-                pixel = y * D.cam.width + x;
-                seed = s == 0u ? tea4(pixel, D.subframe) : D.camseed[pixel];
-                camera_ray(D.cam, x, y, seed, o, d);
-                D.camseed[pixel] = seed;
-                meta = (rq << 16) | ((s % D.K) << 8) | (D.max_depth - 1u);
-            }
-        }
-        uint32_t k = wave_append(valid, next_count);
-        if (valid) {
-            n_o[k] = make_float4(o.x, o.y, o.z, __uint_as_float(pixel));
-            n_d[k] = make_float4(d.x, d.y, d.z, __uint_as_float(seed));
-            n_t[k] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(meta));
+    for (uint32_t j = tid; j < cnt; j += kRegenBlk) {
+        const uint32_t J = first + j;
+        const uint32_t e = find_entry(plan->regen_prefix, nreg, J);
+        const uint32_t rq = plan->regen_req[e];
+        const ReqDesc& D = req[rq];
+        const unsigned long long item = plan->regen_base[e] + (J - plan->regen_prefix[e]);
+        const uint32_t s = (uint32_t)(item / D.npad);
+        const uint32_t i = (uint32_t)(item % D.npad);
+        const uint32_t tsz = D.tw * D.th;
+        const uint32_t tile = i / tsz, w = i % tsz;
+        const uint32_t x = (tile % D.tiles_x) * D.tw + w % D.tw;
+        const uint32_t y = (tile / D.tiles_x) * D.th + w / D.tw;
+        if (x < D.cam.width && y < D.cam.height) {
+            const uint32_t pixel = y * D.cam.width + x;
+            uint32_t seed = s == 0u ? tea4(pixel, D.subframe) : D.camseed[pixel];
+            v3 o, d;
+            camera_ray(D.cam, x, y, seed, o, d);
+            D.camseed[pixel] = seed;
+            const uint32_t meta = (rq << 16) | ((s % D.K) << 8) | (D.max_depth - 1u);
+            n_o[base_off + j] = make_float4(o.x, o.y, o.z, __uint_as_float(pixel));
+            n_d[base_off + j] = make_float4(d.x, d.y, d.z, __uint_as_float(seed));
+            n_t[base_off + j] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(meta));
             paths++;
+        } else {
+            n_t[base_off + j] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kDeadRay));
         }
     }
     paths = wave_sum(paths);
-    if (lane == 0 && paths) atomicAdd(&cnt->paths, (unsigned long long)paths);
+    if (__lane_id() == 0) s_paths[tid >> 6] = paths;
+    __syncthreads();
+    if (tid == 0) {
+        cnt_next[b] = pos0 + cnt;
+        unsigned long long p = 0;
+        for (int w = 0; w < kRegenBlk / 64; w++) p += s_paths[w];
+        bstats[b].paths += p;
+    }
 }
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -477,7 +495,7 @@ size_t request_bytes(const vzLaunchParams& p)
     size_t K = pass_slots(p);
     bool fp64 = (p.flags & VZ_ACCUM_FP64) != 0;
     return align_up(K * 3 * npix * 4, 256) + align_up(3 * npix * (fp64 ? 8 : 4), 256) +
-           align_up(npix * 4, 256) + align_up(K * 4, 256) + 1024;
+           align_up(npix * 4, 256) + align_up(K * 4, 256) + align_up(K * 8, 256) + 1024;
 }
 
 Engine::~Engine()
@@ -489,10 +507,10 @@ Engine::~Engine()
         if (q_d[i]) (void)hipFree(q_d[i]);
         if (q_t[i]) (void)hipFree(q_t[i]);
     }
-    void* dev[] = {shade_q, miss_q, d_req, d_state, d_scenes, d_cnt, d_plan};
+    void* dev[] = {d_bcnt, d_bstats, d_req, d_state, d_scenes, d_cnt, d_plan};
     for (void* ptr : dev)
         if (ptr) (void)hipFree(ptr);
-    void* host[] = {h_cnt, h_state, h_req, h_scenes, h_state_stage, h_status, h_nslots};
+    void* host[] = {h_cnt, h_state, h_req, h_scenes, h_bstats, h_state_stage, h_status, h_nslots};
     for (void* ptr : host)
         if (ptr) (void)hipHostFree(ptr);
     for (auto& b : slot_mem)
@@ -504,8 +522,6 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
     Engine* e = new Engine();
     e->ctx = ctx;
     e->cfg = cfg;
-    e->Q = std::max<uint32_t>(cfg.queue_capacity, 4096u);
-    e->cfg.queue_capacity = e->Q;
     if (e->cfg.tile_w == 0) e->cfg.tile_w = 8;
     if (e->cfg.tile_h == 0) e->cfg.tile_h = 8;
     if (e->cfg.iters_per_graph < 2) e->cfg.iters_per_graph = 2;
@@ -514,6 +530,16 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
         delete e;
         return set_error(s, what);
     };
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_wf_step<false>),
+                                                     kStepBlk, 0) != hipSuccess || occ < 1)
+        occ = 1;
+    uint32_t cus = ctx->num_cus ? ctx->num_cus : 256;
+    e->NB = std::min<uint32_t>(kMaxBlocks, cus * (uint32_t)occ);
+    uint32_t want = std::max<uint32_t>(cfg.queue_capacity, e->NB * 64u);
+    e->R = (uint32_t)align_up((want + e->NB - 1) / e->NB, 64);
+    e->Q = e->NB * e->R;
+    e->cfg.queue_capacity = e->Q;
     size_t Q = e->Q;
     for (int i = 0; i < 2; i++) {
         if (hipMalloc(&e->q_o[i], Q * sizeof(float4)) != hipSuccess ||
@@ -521,8 +547,8 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
             hipMalloc(&e->q_t[i], Q * sizeof(float4)) != hipSuccess)
             return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: ray queue allocation failed");
     }
-    if (hipMalloc(&e->shade_q, Q * sizeof(uint4)) != hipSuccess ||
-        hipMalloc(&e->miss_q, Q * sizeof(uint32_t)) != hipSuccess ||
+    if (hipMalloc(&e->d_bcnt, 2 * e->NB * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&e->d_bstats, e->NB * sizeof(BlockStats)) != hipSuccess ||
         hipMalloc(&e->d_req, kMaxRequests * sizeof(ReqDesc)) != hipSuccess ||
         hipMalloc(&e->d_state, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
         hipMalloc(&e->d_cnt, sizeof(Counters)) != hipSuccess ||
@@ -530,21 +556,18 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
         return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: table allocation failed");
     if (hipHostMalloc(&e->h_cnt, 2 * sizeof(Counters)) != hipSuccess ||
         hipHostMalloc(&e->h_state, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
-        hipHostMalloc(&e->h_req, kMaxRequests * sizeof(ReqDesc)) != hipSuccess)
+        hipHostMalloc(&e->h_req, kMaxRequests * sizeof(ReqDesc)) != hipSuccess ||
+        hipHostMalloc(&e->h_bstats, e->NB * sizeof(BlockStats)) != hipSuccess)
         return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: pinned allocation failed");
     std::memset(e->h_cnt, 0, 2 * sizeof(Counters));
     std::memset(e->h_state, 0, kMaxRequests * sizeof(ReqState));
     std::memset(e->h_req, 0, kMaxRequests * sizeof(ReqDesc));
     if (hipMemset(e->d_state, 0, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
         hipMemset(e->d_cnt, 0, sizeof(Counters)) != hipSuccess ||
-        hipMemset(e->d_plan, 0, sizeof(Plan)) != hipSuccess)
+        hipMemset(e->d_plan, 0, sizeof(Plan)) != hipSuccess ||
+        hipMemset(e->d_bcnt, 0, 2 * e->NB * sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(e->d_bstats, 0, e->NB * sizeof(BlockStats)) != hipSuccess)
         return fail(VZ_ERROR_HIP, "engine: initial memset failed");
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_wf_trace<false>),
-                                                     kBlk, 0) != hipSuccess || occ < 1)
-        occ = 4;
-    uint32_t cus = ctx->num_cus ? ctx->num_cus : 256;
-    e->grid_trace = cus * (uint32_t)occ;
     e->grid_small = cus * 4;
     e->slot_mem.resize(kMaxRequests);
     *out = e;
@@ -553,31 +576,27 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
 
 void engine_destroy(Engine* e) { delete e; }
 
-// Enqueues one wavefront iteration of parity p.
-static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, hipEvent_t ev0,
-                              hipEvent_t ev1)
+// Enqueues one wavefront iteration of parity p (reads slices p, writes 1-p).
+static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, hipEvent_t ev0, hipEvent_t ev1)
 {
     const int n = 1 - p;
+    uint32_t* cnt_cur = e->d_bcnt + (size_t)p * e->NB;
+    uint32_t* cnt_nxt = e->d_bcnt + (size_t)n * e->NB;
     if (ev0) (void)hipEventRecord(ev0, st);
     if (count)
-        hipLaunchKernelGGL(k_wf_trace<true>, dim3(e->grid_trace), dim3(kBlk), 0, st, e->d_scenes,
-                           e->d_req, e->d_state, e->q_o[p], e->q_d[p], e->q_t[p], e->d_cnt, p,
-                           e->shade_q, e->miss_q);
+        hipLaunchKernelGGL(k_wf_step<true>, dim3(e->NB), dim3(kStepBlk), 0, st, e->d_scenes, e->d_req, e->q_o[p],
+                           e->q_d[p], e->q_t[p], e->q_o[n], e->q_d[n], e->q_t[n], cnt_cur, cnt_nxt,
+                           e->d_bstats, e->R);
     else
-        hipLaunchKernelGGL(k_wf_trace<false>, dim3(e->grid_trace), dim3(kBlk), 0, st, e->d_scenes,
-                           e->d_req, e->d_state, e->q_o[p], e->q_d[p], e->q_t[p], e->d_cnt, p,
-                           e->shade_q, e->miss_q);
+        hipLaunchKernelGGL(k_wf_step<false>, dim3(e->NB), dim3(kStepBlk), 0, st, e->d_scenes, e->d_req, e->q_o[p],
+                           e->q_d[p], e->q_t[p], e->q_o[n], e->q_d[n], e->q_t[n], cnt_cur, cnt_nxt,
+                           e->d_bstats, e->R);
     if (ev1) (void)hipEventRecord(ev1, st);
-    hipLaunchKernelGGL(k_wf_miss, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_req, e->q_o[p], e->q_d[p],
-                       e->q_t[p], e->d_cnt, e->miss_q);
-    hipLaunchKernelGGL(k_wf_shade, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_scenes, e->d_req,
-                       e->q_o[p], e->q_d[p], e->q_t[p], e->d_cnt, p, e->shade_q, e->q_o[n], e->q_d[n],
-                       e->q_t[n]);
-    hipLaunchKernelGGL(k_wf_control, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt,
-                       e->d_plan, e->Q, p);
+    hipLaunchKernelGGL(k_wf_control, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt, e->d_plan,
+                       cnt_nxt, e->NB, e->R);
     hipLaunchKernelGGL(k_wf_apply, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_req, e->d_plan);
-    hipLaunchKernelGGL(k_wf_regen, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_req, e->d_plan, e->d_cnt,
-                       p, e->q_o[n], e->q_d[n], e->q_t[n]);
+    hipLaunchKernelGGL(k_wf_regen, dim3(e->NB), dim3(kRegenBlk), 0, st, e->d_req, e->d_plan, cnt_nxt, e->q_o[n],
+                       e->q_d[n], e->q_t[n], e->d_bstats, e->R);
 }
 
 static vzStatus ensure_graph(Engine* e, bool count)
@@ -600,9 +619,8 @@ static vzStatus ensure_graph(Engine* e, bool count)
     return VZ_SUCCESS;
 }
 
-// Fills slot `slot` of the host table for item `it`; allocates its buffers.
-static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const vzLaunchParams& p,
-                           hipStream_t st)
+// Fills slot `slot` of the host table for one request; allocates its buffers.
+static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const vzLaunchParams& p, hipStream_t st)
 {
     ReqDesc& D = e->h_req[slot];
     std::memset(&D, 0, sizeof(D));
@@ -658,7 +676,10 @@ static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const v
     D.camseed = (uint32_t*)base;
     base += align_up(npix * 4, 256);
     D.done = (uint32_t*)base;
+    base += align_up((size_t)D.K * 4, 256);
+    D.segsum = (unsigned long long*)base;
     VZ_HIP_TRY(hipMemsetAsync(D.done, 0, (size_t)D.K * 4, st));
+    VZ_HIP_TRY(hipMemsetAsync(D.segsum, 0, (size_t)D.K * 8, st));
     ReqState& S = e->h_state[slot];
     std::memset(&S, 0, sizeof(S));
     S.active = 1u;
@@ -689,6 +710,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
     uint32_t n = (uint32_t)items.size();
     if (n == 0) return VZ_SUCCESS;
     if (n > kMaxRequests) return set_error(VZ_ERROR_CAPACITY, "too many requests in one batch");
+    if (e->stream_alloc) return set_error(VZ_ERROR_INTERNAL, "engine is in serving mode");
     // the whole table is reset: wait for anything still using it
     VZ_HIP_TRY(hipStreamSynchronize(st));
     std::vector<const vzScene_st*> scenes;
@@ -712,6 +734,8 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
     c0.active = n;
     e->h_cnt[0] = c0;
     VZ_HIP_TRY(hipMemcpyAsync(e->d_cnt, &e->h_cnt[0], sizeof(Counters), hipMemcpyHostToDevice, st));
+    VZ_HIP_TRY(hipMemsetAsync(e->d_bcnt, 0, 2 * e->NB * sizeof(uint32_t), st));
+    VZ_HIP_TRY(hipMemsetAsync(e->d_bstats, 0, e->NB * sizeof(BlockStats), st));
     VZ_HIP_TRY(hipStreamSynchronize(st));  // staging buffers are reused below
 
     const bool use_graph = opt.use_graph && !opt.time_trace;
@@ -796,11 +820,16 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
     Counters fin;
     VZ_HIP_TRY(hipMemcpy(&fin, e->d_cnt, sizeof(Counters), hipMemcpyDeviceToHost));
     VZ_HIP_TRY(hipMemcpy(e->h_state, e->d_state, n * sizeof(ReqState), hipMemcpyDeviceToHost));
+    VZ_HIP_TRY(hipMemcpy(e->h_bstats, e->d_bstats, e->NB * sizeof(BlockStats), hipMemcpyDeviceToHost));
     if (res) {
-        res->segments = fin.segments;
-        res->paths = fin.paths;
-        res->node_visits = fin.node_visits;
-        res->sphere_tests = fin.sphere_tests;
+        *res = RunResult();
+        for (uint32_t b = 0; b < e->NB; b++) {
+            res->segments += e->h_bstats[b].segments;
+            res->hits += e->h_bstats[b].hits;
+            res->node_visits += e->h_bstats[b].node_visits;
+            res->sphere_tests += e->h_bstats[b].sphere_tests;
+            res->paths += e->h_bstats[b].paths;
+        }
         res->render_ms = ms;
         res->trace_ms = trace_ms;
         res->trace_launches = trace_launches;
@@ -838,6 +867,8 @@ vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes)
     std::memset(e->h_status, 0, 2 * kMaxRequests * sizeof(ReqState));
     VZ_HIP_TRY(hipMemsetAsync(e->d_state, 0, kMaxRequests * sizeof(ReqState), st));
     VZ_HIP_TRY(hipMemsetAsync(e->d_cnt, 0, sizeof(Counters), st));
+    VZ_HIP_TRY(hipMemsetAsync(e->d_bcnt, 0, 2 * e->NB * sizeof(uint32_t), st));
+    VZ_HIP_TRY(hipMemsetAsync(e->d_bstats, 0, e->NB * sizeof(BlockStats), st));
     VZ_TRY(ensure_graph(e, false));
     VZ_HIP_TRY(hipStreamSynchronize(st));
     return VZ_SUCCESS;
@@ -859,16 +890,16 @@ vzStatus engine_admit(Engine* e, const RunItem& item, hipStream_t st, uint32_t* 
         sidx = (uint32_t)e->scene_list.size();
         e->scene_list.push_back(item.scene);
         e->h_scenes[sidx] = item.scene->view();
-        VZ_HIP_TRY(hipMemcpyAsync(e->d_scenes + sidx, e->h_scenes + sidx, sizeof(SceneView),
-                                  hipMemcpyHostToDevice, st));
+        VZ_HIP_TRY(hipMemcpyAsync(e->d_scenes + sidx, e->h_scenes + sidx, sizeof(SceneView), hipMemcpyHostToDevice,
+                                  st));
     } else {
         sidx = (uint32_t)(it - e->scene_list.begin());
     }
     VZ_TRY(setup_slot(e, slot, sidx, item.params, st));
     e->h_state_stage[slot] = e->h_state[slot];
     VZ_HIP_TRY(hipMemcpyAsync(e->d_req + slot, e->h_req + slot, sizeof(ReqDesc), hipMemcpyHostToDevice, st));
-    VZ_HIP_TRY(hipMemcpyAsync(e->d_state + slot, e->h_state_stage + slot, sizeof(ReqState),
-                              hipMemcpyHostToDevice, st));
+    VZ_HIP_TRY(hipMemcpyAsync(e->d_state + slot, e->h_state_stage + slot, sizeof(ReqState), hipMemcpyHostToDevice,
+                              st));
     e->slot_used[slot] = 1;
     if (slot + 1 > e->n_slots) {
         e->n_slots = slot + 1;
