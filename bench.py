@@ -91,12 +91,14 @@ def main():
     value = segs / wall / 1e6
     gpu_acc = acc.download(np.float32, (H, W, 4))
 
-    # roofline of the trace kernel.  Algorithmic bytes per segment: ray read
-    # (origin+dir 32 B, request word 4 B), 32 B per BVH node fetched, 16 B per
-    # sphere tested, the queue entry written (16 B shade entry on a hit, 4 B
-    # miss entry otherwise).
-    seg_c, nv_c, ns_c, hit_c = cst.segments, cst.node_visits, cst.sphere_tests, cst.hits
-    bytes_step = 36 * seg_c + 32 * nv_c + 16 * ns_c + 16 * hit_c + 4 * (seg_c - hit_c)
+    # roofline of the step kernel (trace + miss/shade stages, the dominant
+    # kernel).  Algorithmic bytes: per segment the 48 B path record read, 32 B
+    # per BVH node fetched, 16 B per sphere tested; per hit 36 B of sphere,
+    # material and type re-read by the shade stage; per surviving path the
+    # 48 B record written to the next queue; per ended path its 12 B
+    # radiance written to the pass buffer.
+    seg_c, nv_c, ns_c, hit_c, path_c = cst.segments, cst.node_visits, cst.sphere_tests, cst.hits, cst.paths
+    bytes_step = (48 * seg_c + 32 * nv_c + 16 * ns_c + 36 * hit_c + 48 * (seg_c - path_c) + 12 * path_c)
     trace_ms = sum(s.trace_ms for s in stats)
     launches = sum(s.trace_launches for s in stats)
     roof = None
@@ -110,7 +112,7 @@ def main():
             traffic = json.load(open(tf)).get("traffic_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_wf_trace", "launches": launches, "avg_launch_ms": round(avg_ms, 4),
+                "kernel": "k_wf_step", "launches": launches, "avg_launch_ms": round(avg_ms, 4),
                 "trace_share_of_step": round(trace_ms / (sum(s.render_ms for s in stats)), 3),
                 "bytes_per_segment": round(bytes_step / seg_c, 1),
                 "node_visits_per_segment": round(nv_c / seg_c, 2),

@@ -85,7 +85,7 @@ struct Plan {
 
 struct EngineConfig {
     uint32_t queue_capacity = 1u << 22;
-    uint32_t tile_w = 8, tile_h = 8;
+    uint32_t tile_w = 32, tile_h = 2;   // one wave of camera rays = 32x2 pixels
     uint32_t iters_per_graph = 16;
 };
 
@@ -118,7 +118,7 @@ struct Engine {
     uint32_t grid_small = 0;
     hipGraphExec_t graph = nullptr;
     hipStream_t graph_stream = nullptr;
-    bool graph_count = false;
+    int graph_key = -1;
     // per-slot buffers, reused when large enough
     std::vector<DeviceBlock> slot_mem;
     uint64_t slot_bytes_total = 0;

@@ -16,6 +16,7 @@ namespace {
 constexpr int kStepBlk = 512;   // step-kernel workgroup (8 waves)
 constexpr int kStackS = 8;      // LDS stack slots per lane
 constexpr int kTbl = 128;       // LDS (request, pass) tally table entries
+constexpr uint32_t kLdsNodes = 512;  // scenes up to this many nodes are staged in LDS
 constexpr int kRegenBlk = 256;
 constexpr int kBlk = 256;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
@@ -60,7 +61,7 @@ __device__ __forceinline__ void terminate(const ReqDesc& D, uint32_t slot, uint3
 }
 
 // ---- step: trace + miss/shade stages + compaction, one slice per workgroup ----
-template <bool COUNT>
+template <bool COUNT, bool LDSN>
 __global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restrict__ scenes,
                                                       const ReqDesc* __restrict__ req,
                                                       const float4* __restrict__ q_o,
@@ -78,7 +79,13 @@ __global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restric
     __shared__ unsigned long long t_seg[kTbl];
     __shared__ uint32_t s_head, s_out;
     __shared__ unsigned long long s_red[4][kStepBlk / 64];
+    __shared__ uint4 s_bvh[LDSN ? 2 * kLdsNodes : 1];
     const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+    if (LDSN) {  // one small scene: stage its node array in LDS
+        const uint32_t nn = 2u * (scenes[0].n - 1u);
+        const uint4* src = scenes[0].nodes;
+        for (uint32_t i = tid; i < nn; i += kStepBlk) s_bvh[i] = src[i];
+    }
     for (uint32_t i = tid; i < (uint32_t)kTbl; i += kStepBlk) {
         t_key[i] = kEmpty;
         t_cnt[i] = 0u;
@@ -99,26 +106,53 @@ __global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restric
         c0 = bcast(c0);
         if (c0 >= count) break;
         const uint32_t local = c0 + lane;
+        const size_t qi = base_off + local;
         bool valid = local < count;
-        float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4, t4 = o4;
-        uint32_t meta = kDeadRay;
+        uint32_t meta = kDeadRay, sidx = 0xFFFFFFFFu;
         if (valid) {
-            o4 = q_o[base_off + local];
-            d4 = q_d[base_off + local];
-            t4 = q_t[base_off + local];
-            meta = __float_as_uint(t4.w);
+            meta = __float_as_uint(q_t[qi].w);
             valid = meta != kDeadRay;
+            if (valid) sidx = req[meta >> 16].scene;
+        }
+        // trace stage: only the ray is live across the traversal; the path
+        // state is re-read afterwards.  A wave whose rays share one scene
+        // (every plain launch) reads the scene descriptor into scalars.
+        TraceResult r{0.0f, 0xFFFFFFFFu};
+        {
+            float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
+            if (valid) {
+                o4 = q_o[qi];
+                d4 = q_d[qi];
+            }
+            const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+            const uint64_t vm = __ballot(valid);
+            const uint32_t s0 = __shfl(sidx, vm ? __ffsll((unsigned long long)vm) - 1 : 0);
+            if (LDSN) {
+                const SceneView sc = scenes[0];
+                if (valid)
+                    r = closest_hit<kStackS, COUNT>(sc, LdsNodes{s_bvh}, o, d, 0.001f, 1e16f, s_node + tid, s_t + tid,
+                                                    kStepBlk, nv, ns);
+            } else if (__ballot(valid && sidx != s0) == 0ull) {
+                const SceneView sc = scenes[bcast(s0)];
+                if (valid)
+                    r = closest_hit<kStackS, COUNT>(sc, GlobalNodes{sc.nodes}, o, d, 0.001f, 1e16f, s_node + tid,
+                                                    s_t + tid, kStepBlk, nv, ns);
+            } else if (valid) {
+                const SceneView sc = scenes[sidx];
+                r = closest_hit<kStackS, COUNT>(sc, GlobalNodes{sc.nodes}, o, d, 0.001f, 1e16f, s_node + tid,
+                                                s_t + tid, kStepBlk, nv, ns);
+            }
         }
         bool ended = false, survive = false;
         uint32_t key = 0, pathseg = 0, seed = 0;
         v3 pos = mk(0.0f, 0.0f, 0.0f), nd = pos, thr = pos;
+        float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (valid) {
+            o4 = q_o[qi];
+            const float4 d4 = q_d[qi], t4 = q_t[qi];
             const uint32_t rq = meta >> 16, slot = (meta >> 8) & 0xFFu, depth = meta & 0xFFu;
             const ReqDesc* D = &req[rq];
-            const SceneView sc = scenes[D->scene];
             const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
-            const TraceResult r = closest_hit<kStackS, COUNT>(sc, o, d, 0.001f, 1e16f, s_node + tid, s_t + tid,
-                                                              kStepBlk, nv, ns);
             seg++;
             seed = __float_as_uint(d4.w);
             v3 L = mk(0.0f, 0.0f, 0.0f);
@@ -130,6 +164,7 @@ __global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restric
                 nh++;
                 if (depth > 0u) {
                     // shade stage (RayTracer.cu:272-440)
+                    const SceneView sc = scenes[sidx];
                     const float4 s = sc.spheres[r.leaf];
                     const uint32_t type = sc.leaf_info[r.leaf] >> 30;
                     const float4 mat = sc.mats[r.leaf];
@@ -531,9 +566,13 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
         return set_error(s, what);
     };
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_wf_step<false>),
+    int occ_lds = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_wf_step<false, false>),
                                                      kStepBlk, 0) != hipSuccess || occ < 1)
         occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_lds, reinterpret_cast<const void*>(k_wf_step<false, true>),
+                                                     kStepBlk, 0) == hipSuccess && occ_lds >= 1)
+        occ = std::min(occ, occ_lds);
     uint32_t cus = ctx->num_cus ? ctx->num_cus : 256;
     e->NB = std::min<uint32_t>(kMaxBlocks, cus * (uint32_t)occ);
     uint32_t want = std::max<uint32_t>(cfg.queue_capacity, e->NB * 64u);
@@ -577,20 +616,22 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
 void engine_destroy(Engine* e) { delete e; }
 
 // Enqueues one wavefront iteration of parity p (reads slices p, writes 1-p).
-static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, hipEvent_t ev0, hipEvent_t ev1)
+static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool lds, hipEvent_t ev0,
+                              hipEvent_t ev1)
 {
     const int n = 1 - p;
     uint32_t* cnt_cur = e->d_bcnt + (size_t)p * e->NB;
     uint32_t* cnt_nxt = e->d_bcnt + (size_t)n * e->NB;
     if (ev0) (void)hipEventRecord(ev0, st);
-    if (count)
-        hipLaunchKernelGGL(k_wf_step<true>, dim3(e->NB), dim3(kStepBlk), 0, st, e->d_scenes, e->d_req, e->q_o[p],
-                           e->q_d[p], e->q_t[p], e->q_o[n], e->q_d[n], e->q_t[n], cnt_cur, cnt_nxt,
-                           e->d_bstats, e->R);
-    else
-        hipLaunchKernelGGL(k_wf_step<false>, dim3(e->NB), dim3(kStepBlk), 0, st, e->d_scenes, e->d_req, e->q_o[p],
-                           e->q_d[p], e->q_t[p], e->q_o[n], e->q_d[n], e->q_t[n], cnt_cur, cnt_nxt,
-                           e->d_bstats, e->R);
+#define VZ_STEP(C, L)                                                                                      \
+    hipLaunchKernelGGL((k_wf_step<C, L>), dim3(e->NB), dim3(kStepBlk), 0, st, e->d_scenes, e->d_req, e->q_o[p], \
+                       e->q_d[p], e->q_t[p], e->q_o[n], e->q_d[n], e->q_t[n], cnt_cur, cnt_nxt, e->d_bstats, e->R)
+    if (count) {
+        if (lds) VZ_STEP(true, true); else VZ_STEP(true, false);
+    } else {
+        if (lds) VZ_STEP(false, true); else VZ_STEP(false, false);
+    }
+#undef VZ_STEP
     if (ev1) (void)hipEventRecord(ev1, st);
     hipLaunchKernelGGL(k_wf_control, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt, e->d_plan,
                        cnt_nxt, e->NB, e->R);
@@ -599,9 +640,10 @@ static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, hipE
                        e->q_d[n], e->q_t[n], e->d_bstats, e->R);
 }
 
-static vzStatus ensure_graph(Engine* e, bool count)
+static vzStatus ensure_graph(Engine* e, bool count, bool lds)
 {
-    if (e->graph && e->graph_count == count) return VZ_SUCCESS;
+    const int key = (count ? 1 : 0) | (lds ? 2 : 0);
+    if (e->graph && e->graph_key == key) return VZ_SUCCESS;
     if (e->graph) {
         (void)hipGraphExecDestroy(e->graph);
         e->graph = nullptr;
@@ -609,13 +651,13 @@ static vzStatus ensure_graph(Engine* e, bool count)
     if (!e->graph_stream) VZ_HIP_TRY(hipStreamCreateWithFlags(&e->graph_stream, hipStreamNonBlocking));
     VZ_HIP_TRY(hipStreamBeginCapture(e->graph_stream, hipStreamCaptureModeThreadLocal));
     for (uint32_t it = 0; it < e->cfg.iters_per_graph; it++)
-        enqueue_iteration(e, e->graph_stream, (int)(it & 1u), count, nullptr, nullptr);
+        enqueue_iteration(e, e->graph_stream, (int)(it & 1u), count, lds, nullptr, nullptr);
     hipGraph_t g = nullptr;
     VZ_HIP_TRY(hipStreamEndCapture(e->graph_stream, &g));
     hipError_t err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (err != hipSuccess) return set_error(VZ_ERROR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
-    e->graph_count = count;
+    e->graph_key = key;
     return VZ_SUCCESS;
 }
 
@@ -739,7 +781,9 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
     VZ_HIP_TRY(hipStreamSynchronize(st));  // staging buffers are reused below
 
     const bool use_graph = opt.use_graph && !opt.time_trace;
-    if (use_graph) VZ_TRY(ensure_graph(e, opt.count_visits));
+    // a single small scene is staged in LDS by every step workgroup
+    const bool lds = scenes.size() == 1 && scenes[0]->n >= 2 && scenes[0]->n - 1 <= kLdsNodes;
+    if (use_graph) VZ_TRY(ensure_graph(e, opt.count_visits, lds));
     const uint32_t B = e->cfg.iters_per_graph;
     hipEvent_t t0, t1, blk_ev[2];
     VZ_HIP_TRY(hipEventCreate(&t0));
@@ -762,7 +806,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
             for (uint32_t it = 0; it < B; it++) {
                 hipEvent_t a = opt.time_trace ? tev[k & 1][2 * it] : nullptr;
                 hipEvent_t b = opt.time_trace ? tev[k & 1][2 * it + 1] : nullptr;
-                enqueue_iteration(e, st, (int)(it & 1u), opt.count_visits, a, b);
+                enqueue_iteration(e, st, (int)(it & 1u), opt.count_visits, lds, a, b);
             }
             VZ_HIP_TRY(hipGetLastError());
         }
@@ -869,7 +913,7 @@ vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes)
     VZ_HIP_TRY(hipMemsetAsync(e->d_cnt, 0, sizeof(Counters), st));
     VZ_HIP_TRY(hipMemsetAsync(e->d_bcnt, 0, 2 * e->NB * sizeof(uint32_t), st));
     VZ_HIP_TRY(hipMemsetAsync(e->d_bstats, 0, e->NB * sizeof(BlockStats), st));
-    VZ_TRY(ensure_graph(e, false));
+    VZ_TRY(ensure_graph(e, false, false));
     VZ_HIP_TRY(hipStreamSynchronize(st));
     return VZ_SUCCESS;
 }

@@ -34,7 +34,7 @@ __device__ __forceinline__ v3 trace_path(const SceneView& sc, v3 o, v3 d, uint32
     int depth = (int)max_depth - 1;
     for (;;) {
         seg++;
-        TraceResult r = closest_hit<S, COUNT>(sc, o, d, 0.001f, 1e16f, ln, lt, kMegaBlk, nv, ns);
+        TraceResult r = closest_hit<S, COUNT>(sc, GlobalNodes{sc.nodes}, o, d, 0.001f, 1e16f, ln, lt, kMegaBlk, nv, ns);
         if (r.leaf == 0xFFFFFFFFu) return mul(thr, sky(d));
         if (depth <= 0) return mk(0.0f, 0.0f, 0.0f);
         float4 s = sc.spheres[r.leaf];

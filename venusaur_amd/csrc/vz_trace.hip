@@ -28,7 +28,7 @@ __global__ void __launch_bounds__(kTraceBlk) k_trace(SceneView sc, const vzRay* 
     if (i < n) {
         const float4* rp = reinterpret_cast<const float4*>(rays + i);
         float4 a = rp[0], b = rp[1];
-        TraceResult r = closest_hit<S, COUNT>(sc, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w,
+        TraceResult r = closest_hit<S, COUNT>(sc, GlobalNodes{sc.nodes}, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w,
                                               s_node + threadIdx.x, s_t + threadIdx.x, kTraceBlk, nv, ns);
         uint32_t prim = r.leaf == 0xFFFFFFFFu ? VZ_MISS : (sc.leaf_info[r.leaf] & 0x3FFFFFFFu);
         reinterpret_cast<uint2*>(hits)[i] = make_uint2(__float_as_uint(r.t), prim);

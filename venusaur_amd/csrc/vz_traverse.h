@@ -24,12 +24,31 @@ struct TraceResult {
     uint32_t leaf;  // leaf (Morton-order) index, 0xFFFFFFFF on a miss
 };
 
+// Where the 32-byte nodes come from: the scene's array in HBM, or a copy a
+// workgroup staged in LDS (small scenes).
+struct GlobalNodes {
+    const uint4* p;
+    VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b) const
+    {
+        a = p[2 * i];
+        b = p[2 * i + 1];
+    }
+};
+struct LdsNodes {
+    const uint4* p;  // points into __shared__ memory
+    VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b) const
+    {
+        a = p[2 * i];
+        b = p[2 * i + 1];
+    }
+};
+
 // S: LDS slots per lane (power of two).  lds_node/lds_t point at this lane's
 // column; consecutive slots are `stride` words apart.
-template <int S, bool COUNT>
-VZ_DEV TraceResult closest_hit(const SceneView& sc, v3 o, v3 d, float tmin, float tmax,
-                               uint32_t* lds_node, float* lds_t, uint32_t stride, uint32_t& nv,
-                               uint32_t& ns)
+template <int S, bool COUNT, class NodeSrc>
+VZ_DEV TraceResult closest_hit(const SceneView& sc, const NodeSrc& nodes, v3 o, v3 d, float tmin,
+                               float tmax, uint32_t* lds_node, float* lds_t, uint32_t stride,
+                               uint32_t& nv, uint32_t& ns)
 {
     float best = tmax;
     uint32_t bleaf = 0xFFFFFFFFu, bprim = 0xFFFFFFFFu;
@@ -58,8 +77,8 @@ VZ_DEV TraceResult closest_hit(const SceneView& sc, v3 o, v3 d, float tmin, floa
     uint32_t budget = sc.n;
     for (;;) {
         if (budget-- == 0u) break;
-        const uint4 A = sc.nodes[2 * node];
-        const uint4 B = sc.nodes[2 * node + 1];
+        uint4 A, B;
+        nodes.fetch(node, A, B);
         if (COUNT) nv++;
         const float no[3] = {__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z)};
         const uint32_t meta = A.w;
