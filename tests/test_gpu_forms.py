@@ -1,0 +1,145 @@
+"""GPU tests of the batched and serving forms: every request's result is
+bit-identical to the CPU oracle rendering that request alone, whatever else
+shares the wavefront with it."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from util import GpuTarget, book_lookat, box_lookat, gpu_params, oracle_params
+from venusaur_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def scenes(ctx):
+    bs, bm = api.book_scene()
+    out = {"book": (api.Scene(ctx, bs, bm), bs, bm)}
+    for n, seed in ((2000, 41), (12000, 42)):
+        s, m = api.box_scene(n, seed)
+        out[f"box{n}"] = (api.Scene(ctx, s, m), s, m)
+    yield out
+    for v in out.values():
+        v[0].close()
+
+
+# (scene, w, h, spp, depth, fp64)
+MIX = [("book", 64, 48, 8, 50, False), ("box2000", 96, 64, 4, 16, False), ("box12000", 80, 80, 16, 16, False),
+       ("book", 48, 32, 24, 16, True), ("box2000", 40, 30, 2, 4, False), ("box12000", 33, 17, 3, 16, False)]
+
+
+def _cam(name, w, h):
+    return book_lookat(w, h) if name == "book" else box_lookat(w, h)
+
+
+def _oracle(sc, w, h, spp, depth, fp64, name, obvh):
+    s, m = sc[1], sc[2]
+    acc, _, st = O.render(s, m, oracle_params(w, h, spp, depth, _cam(name, w, h), accum_fp64=fp64), bvh=obvh)
+    return acc, st
+
+
+@pytest.fixture(scope="module")
+def oracle_results(scenes):
+    bvhs = {k: O.BVH(v[1]) for k, v in scenes.items()}
+    return [_oracle(scenes[n], w, h, spp, d, f, n, bvhs[n]) for n, w, h, spp, d, f in MIX]
+
+
+def _items(scenes):
+    items, tgts = [], []
+    for n, w, h, spp, d, f in MIX:
+        t = GpuTarget(w, h, fp64=f)
+        p = t.bind(gpu_params(w, h, spp, d, _cam(n, w, h), flags=api.ACCUM_FP64 if f else 0))
+        items.append((scenes[n][0], p))
+        tgts.append(t)
+    return items, tgts
+
+
+def test_batched_alone_equals_mix(ctx, scenes, oracle_results):
+    say("case: batched mix of 6 requests")
+    items, tgts = _items(scenes)
+    st = api.batch_launch(ctx, items)
+    for k, (t, (want, ost)) in enumerate(zip(tgts, oracle_results)):
+        assert np.array_equal(t.accum_host(), want), MIX[k]
+    assert st.segments == sum(o[1].segments for o in oracle_results)
+    # replay: the captured graph again, same results
+    for t in tgts:
+        t.accum.zero()
+    api.batch_launch(ctx, items)
+    for k, (t, (want, _)) in enumerate(zip(tgts, oracle_results)):
+        assert np.array_equal(t.accum_host(), want), MIX[k]
+
+
+@pytest.mark.parametrize("q,tw,th", [(1 << 16, 8, 8), (1 << 20, 16, 4), (1 << 22, 64, 1)])
+def test_batched_any_tuning_same_result(ctx, scenes, oracle_results, q, tw, th):
+    say(f"case: batched tuning q={q} tile={tw}x{th}")
+    items, tgts = _items(scenes)
+    t = api.Tuning()
+    t.queue_capacity, t.tile_w, t.tile_h, t.iters_per_graph = q, tw, th, 8
+    api.batch_launch(ctx, items, tuning=t)
+    for k, (tg, (want, _)) in enumerate(zip(tgts, oracle_results)):
+        assert np.array_equal(tg.accum_host(), want), MIX[k]
+
+
+def test_serving_continuous_batching(ctx, scenes, oracle_results):
+    say("case: serving, staggered submissions")
+    srv = api.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64)
+    items, tgts = _items(scenes)
+    ids = []
+    for k, (sc, p) in enumerate(items):
+        ids.append(srv.submit(sc, p))
+        if k == 2:
+            time.sleep(0.01)  # later requests arrive while the first ones run
+    # a second copy of the whole mix, submitted while the first is in flight
+    items2, tgts2 = _items(scenes)
+    ids2 = [srv.submit(sc, p) for sc, p in items2]
+    srv.drain()
+    res = [srv.poll(i) for i in ids + ids2]
+    assert all(r is not None and r.status == 0 for r in res)
+    for k, (t, (want, ost)) in enumerate(zip(tgts + tgts2, oracle_results + oracle_results)):
+        assert np.array_equal(t.accum_host(), want), k
+        assert res[k].segments == ost.segments
+        assert res[k].submit_ms <= res[k].admit_ms <= res[k].finish_ms
+    srv.close()
+
+
+def test_serving_admission_by_footprint(ctx, scenes):
+    say("case: serving admission budget")
+    w, h, spp = 96, 64, 4
+    p0 = gpu_params(w, h, spp, 16, box_lookat(w, h))
+    fp = api.footprint(p0)
+    srv = api.Server(ctx, hbm_budget_bytes=int(fp * 1.5), max_active=16)
+    tg = [GpuTarget(w, h) for _ in range(3)]
+    ids = [srv.submit(scenes["box2000"][0], t.bind(gpu_params(w, h, spp, 16, box_lookat(w, h)))) for t in tg]
+    big = GpuTarget(512, 512)
+    big_id = srv.submit(scenes["box2000"][0], big.bind(gpu_params(512, 512, 4, 16, box_lookat(512, 512))))
+    srv.drain()
+    r = [srv.poll(i) for i in ids]
+    assert all(x.status == 0 for x in r)
+    # only one request fits the budget at a time: admissions are serialised
+    assert r[1].admit_ms >= r[0].finish_ms and r[2].admit_ms >= r[1].finish_ms
+    assert srv.poll(big_id).status == 5  # larger than the whole budget
+    first = tg[0].accum_host()
+    assert all(np.array_equal(t.accum_host(), first) for t in tg[1:])
+    srv.close()
+
+
+def test_autotune_picks_a_candidate_and_keeps_results(ctx, scenes, oracle_results):
+    say("case: autotune")
+    items, tgts = _items(scenes)
+    t = api.autotune(ctx, items)
+    assert t.queue_capacity in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+    assert (t.tile_w, t.tile_h) in ((8, 8), (16, 4), (32, 2), (64, 1))
+    for tg in tgts:
+        tg.accum.zero()
+    api.batch_launch(ctx, items, tuning=t)
+    for k, (tg, (want, _)) in enumerate(zip(tgts, oracle_results)):
+        assert np.array_equal(tg.accum_host(), want), MIX[k]
+    assert ctx.default_tuning().queue_capacity == t.queue_capacity
