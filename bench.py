@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--depth", type=int, default=50)
     ap.add_argument("--cpu-row-step", type=int, default=8, help="CPU baseline renders every k-th row")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -57,7 +59,14 @@ def main():
         import __graft_entry__
         __graft_entry__.build()
     from venusaur_amd import api
-    from venusaur_amd.hipmem import DeviceBuffer, synchronize
+    from venusaur_amd.hipmem import DeviceBuffer, copy_d2d, synchronize
+
+    if a.calibrate_copy:
+        nb = a.calibrate_copy << 20
+        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
+        copy_d2d(dst, src, nb)
+        src.free()
+        dst.free()
 
     W, H, SPP, DEPTH = a.width, a.height, a.spp, a.depth
     aspect = float(np.float32(W) / np.float32(H))
@@ -106,12 +115,18 @@ def main():
         per_launch_bytes = bytes_step * a.steps / launches
         avg_ms = trace_ms / launches
         achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-        traffic = None
-        tf = os.path.join(ROOT, "profiles", "trace_kernel_traffic.json")
-        if os.path.exists(tf):
+        # measured HBM bytes per launch of the same kernel on the same
+        # workload: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy
+        # kernel of known size (profiles/k_wf_step_book_1080p_256spp.md)
+        traffic = measured_gbs = None
+        tf = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
+        if os.path.exists(tf) and (W, H, SPP, DEPTH) == (1920, 1080, 256, 50):
             traffic = json.load(open(tf)).get("traffic_bytes_per_launch")
+            measured_gbs = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "measured_hbm_gbs": measured_gbs,
+                "algorithmic_bytes_per_launch": round(per_launch_bytes),
                 "kernel": "k_wf_step", "launches": launches, "avg_launch_ms": round(avg_ms, 4),
                 "trace_share_of_step": round(trace_ms / (sum(s.render_ms for s in stats)), 3),
                 "bytes_per_segment": round(bytes_step / seg_c, 1),

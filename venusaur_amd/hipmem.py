@@ -57,6 +57,11 @@ def synchronize():
     _ck(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+def copy_d2d(dst, src, nbytes):
+    """Device-to-device copy (the runtime's copy kernel), synchronous."""
+    _ck(hip().hipMemcpy(dst.ptr, src.ptr, int(nbytes), D2D), "hipMemcpy D2D")
+
+
 class DeviceBuffer:
     def __init__(self, nbytes):
         self.nbytes = int(nbytes)
