@@ -45,7 +45,142 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (scales rocprof FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--batch-scenes", type=int, default=16, help="scenes in the batched-form comparison (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=120, help="requests in the serving comparison (0 = skip)")
     return ap.parse_args()
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _build_scenes(ctx, reqs):
+    from venusaur_amd import api
+    scenes, build_ms = {}, []
+    for r in reqs:
+        if r.scene_key not in scenes:
+            s, m = api.box_scene(r.n_small, 1000 + r.scene_key)
+            scenes[r.scene_key] = api.Scene(ctx, s, m)
+            build_ms.append(scenes[r.scene_key].info().build_ms)
+    return scenes, build_ms
+
+
+def run_batched(ctx, n_scenes):
+    """Batched form vs sequential plain launches of the same 64-scene mix."""
+    from venusaur_amd import api, workloads
+    from venusaur_amd.hipmem import synchronize
+    reqs = workloads.batch_requests(n_scenes)
+    scenes, build_ms = _build_scenes(ctx, reqs)
+    for r in reqs:
+        r.bind()
+    items = [(scenes[r.scene_key], r.params) for r in reqs]
+    synchronize()
+    t0 = time.perf_counter()
+    segs_plain = sum(scenes[r.scene_key].launch(r.params).segments for r in reqs)
+    t_plain = time.perf_counter() - t0
+    want = [_sha(r.accum_host()) for r in reqs]
+    tuning = api.autotune(ctx, items)  # untimed: picks queue capacity and tile shape
+    api.batch_launch(ctx, items, tuning=tuning)  # warm the engine and its graph
+    synchronize()
+    t0 = time.perf_counter()
+    st = api.batch_launch(ctx, items, tuning=tuning)
+    t_batch = time.perf_counter() - t0
+    same = all(_sha(r.accum_host()) == w for r, w in zip(reqs, want))
+    out = {"workload": f"{n_scenes} box scenes (1e3-2e5 spheres), 256-1024 px square, 4/16/64 spp, depth 16",
+           "segments": int(st.segments), "plain_sequential_mseg_s": round(segs_plain / t_plain / 1e6, 1),
+           "batched_mseg_s": round(st.segments / t_batch / 1e6, 1),
+           "speedup_vs_plain": round(t_plain / t_batch, 2), "plain_s": round(t_plain, 3),
+           "batched_s": round(t_batch, 3), "bit_identical_to_plain": same, "segments_match": segs_plain == st.segments,
+           "tuning": tuning.as_dict(), "lbvh_build_ms_mean": round(float(np.mean(build_ms)), 3)}
+    for sc in scenes.values():
+        sc.close()
+    return out
+
+
+def run_served(ctx, n_requests, rmse_samples=3):
+    """Serving form vs one-at-a-time plain launches on the same Poisson stream."""
+    import threading
+    from venusaur_amd import api, workloads
+    from venusaur_amd.hipmem import synchronize
+    import oracle as O
+    # calibrate the plain capacity on the stream's own mix, then overload it 3x
+    probe = workloads.request_stream(24, 1.0, seed=12)
+    pscenes, _ = _build_scenes(ctx, probe)
+    for r in probe:
+        r.bind()
+    synchronize()
+    t0 = time.perf_counter()
+    for r in probe:
+        pscenes[r.scene_key].launch(r.params)
+    cap = len(probe) / (time.perf_counter() - t0)
+    for sc in pscenes.values():
+        sc.close()
+    rate = 3.0 * cap
+    reqs = workloads.request_stream(n_requests, rate, seed=11)
+    scenes, build_ms = _build_scenes(ctx, reqs)
+    for r in reqs:
+        r.bind()
+    synchronize()
+    # plain: one request at a time in arrival order
+    lat_plain, segs = [], 0
+    t_start = time.perf_counter()
+    for r in reqs:
+        now = time.perf_counter() - t_start
+        if now < r.arrival_s:
+            time.sleep(r.arrival_s - now)
+        segs += scenes[r.scene_key].launch(r.params).segments
+        lat_plain.append(time.perf_counter() - t_start - r.arrival_s)
+    t_plain_end = time.perf_counter() - t_start
+    want = [_sha(r.accum_host()) for r in reqs]
+    # served: a client thread submits at the arrival times
+    srv = api.Server(ctx, hbm_budget_bytes=64 << 30, max_active=512, tuning=ctx.default_tuning())
+    ids = [None] * len(reqs)
+    t_srv0 = time.perf_counter()  # the server's clock starts at creation (same monotonic clock)
+    t_start = time.perf_counter()
+
+    def client():
+        for k, r in enumerate(reqs):
+            now = time.perf_counter() - t_start
+            if now < r.arrival_s:
+                time.sleep(r.arrival_s - now)
+            ids[k] = srv.submit(scenes[r.scene_key], r.params)
+    th = threading.Thread(target=client)
+    th.start()
+    th.join()
+    srv.drain()
+    res = [srv.poll(i) for i in ids]
+    off = t_start - t_srv0
+    lat_srv = [(x.finish_ms / 1e3 - off) - r.arrival_s for x, r in zip(res, reqs)]
+    t_srv_end = max(x.finish_ms for x in res) / 1e3 - off
+    same = all(_sha(r.accum_host()) == w for r, w in zip(reqs, want))
+    ok = all(x.status == 0 for x in res)
+    srv.close()
+    # RMSE against the FP64 oracle on the smallest requests
+    small = sorted(range(len(reqs)), key=lambda k: (reqs[k].paths * reqs[k].n_small))[:rmse_samples]
+    rmse64 = {}
+    for k in small:
+        r = reqs[k]
+        s, m = O.box_scene(r.n_small, 1000 + r.scene_key)
+        cam = O.camera_lookat(list(r.lookfrom), list(r.lookat), r.vfov, float(np.float32(r.width) / np.float32(r.height)),
+                              0.0, 10.0)
+        p = O.make_params(r.width, r.height, r.spp, r.depth, cam)
+        acc64, _ = O.render_f64(s, m, p, bvh=O.BVH(s))
+        got = r.accum_host()
+        d = got[..., :3].astype(np.float64) - acc64[..., :3]
+        rmse64[f"req{k}_{r.width}px_{r.spp}spp_{r.n_small}sph"] = float(np.sqrt(np.mean(d * d)))
+    for sc in scenes.values():
+        sc.close()
+    pct = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 1)  # noqa: E731
+    return {"workload": f"{n_requests} Poisson requests at {rate:.1f}/s (3x the measured plain capacity of "
+                        f"{cap:.1f}/s), 20% camera reuse of resident scenes, 5% 512x512 at 2048 spp FP64",
+            "segments": int(segs), "plain_mseg_s": round(segs / t_plain_end / 1e6, 1),
+            "served_mseg_s": round(segs / t_srv_end / 1e6, 1),
+            "speedup_vs_plain": round(t_plain_end / t_srv_end, 2),
+            "plain_p50_ms": pct(lat_plain, 50), "plain_p99_ms": pct(lat_plain, 99),
+            "served_p50_ms": pct(lat_srv, 50), "served_p99_ms": pct(lat_srv, 99),
+            "bit_identical_to_plain": same, "all_ok": ok, "rmse_vs_fp64_oracle": rmse64,
+            "scenes_resident": len(scenes), "lbvh_build_ms_mean": round(float(np.mean(build_ms)), 3)}
 
 
 def main():
@@ -166,6 +301,12 @@ def main():
             (cacc[rows64, :, :3].astype(np.float64) - acc64[rows64, :, :3]) ** 2)))
         checks["fp64_sample"] = f"rows y%{r64}==0"
 
+    forms = {}
+    if a.batch_scenes:
+        forms["batched"] = run_batched(ctx, a.batch_scenes)
+    if a.serve_requests:
+        forms["served"] = run_served(ctx, a.serve_requests)
+
     info = ctx.info()
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mray-segments/s", "n_gpus": 1,
@@ -175,7 +316,7 @@ def main():
         "config": {"workload": f"book scene {W}x{H} {SPP}spp depth {DEPTH} plain ({a.mode})",
                    "spheres": int(len(spheres)), "width": W, "height": H, "spp": SPP,
                    "max_depth": DEPTH, "device": info["name"], "cus": info["num_cus"]},
-        "roofline": roof, "cpu_baseline": cpu, "checks": checks,
+        "roofline": roof, "cpu_baseline": cpu, "checks": checks, "forms": forms,
     }
     print(json.dumps(out), flush=True)
     scene.close()
