@@ -148,7 +148,9 @@ typedef struct vzTuning {
     uint32_t queue_capacity;  /* rays in flight (0 = default) */
     uint32_t tile_w, tile_h;  /* camera-ray tile: tile_w*tile_h pixels per block of items */
     uint32_t iters_per_graph; /* wavefront iterations captured per hipGraph (even) */
-    uint32_t stack_lds;       /* reserved */
+    uint32_t refill_lanes;    /* a wave fetches new rays once fewer than this many of its lanes
+                                 still trace: 1 = only when all are done, 65 = after every
+                                 node step, 0 = the default for the scene's node placement */
 } vzTuning;
 
 /* Renders every item in one persistent wavefront captured as a hipGraph;

@@ -60,6 +60,7 @@ static vzStatus get_engine(vzContext ctx, Engine** out)
             cfg.tile_w = ctx->tuning.tile_w;
             cfg.tile_h = ctx->tuning.tile_h;
             cfg.iters_per_graph = ctx->tuning.iters_per_graph;
+            cfg.refill_lanes = ctx->tuning.refill_lanes;
         }
         VZ_TRY(engine_create(ctx, cfg, &ctx->engine));
     }
@@ -340,14 +341,14 @@ vzStatus vzDefaultTuning(vzContext ctx, vzTuning* out)
     out->tile_w = cfg.tile_w;
     out->tile_h = cfg.tile_h;
     out->iters_per_graph = cfg.iters_per_graph;
-    out->stack_lds = 8;
+    out->refill_lanes = cfg.refill_lanes;
     return VZ_SUCCESS;
 }
 
 static bool same_tuning(const EngineConfig& c, const vzTuning& t)
 {
     return c.queue_capacity == t.queue_capacity && c.tile_w == t.tile_w && c.tile_h == t.tile_h &&
-           c.iters_per_graph == t.iters_per_graph;
+           c.iters_per_graph == t.iters_per_graph && c.refill_lanes == t.refill_lanes;
 }
 
 vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n, const vzTuning* tuning, vzStream stream,
@@ -369,9 +370,10 @@ vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n, cons
     vzTuning t;
     if (tuning) t = *tuning; else VZ_TRY(vzDefaultTuning(ctx, &t));
     if (t.queue_capacity == 0) t.queue_capacity = EngineConfig().queue_capacity;
-    if (t.tile_w == 0) t.tile_w = 8;
-    if (t.tile_h == 0) t.tile_h = 8;
-    if (t.iters_per_graph < 2) t.iters_per_graph = 16;
+    if (t.tile_w == 0) t.tile_w = EngineConfig().tile_w;
+    if (t.tile_h == 0) t.tile_h = EngineConfig().tile_h;
+    if (t.iters_per_graph < 2) t.iters_per_graph = EngineConfig().iters_per_graph;
+    if (t.refill_lanes > 65) t.refill_lanes = 65;
     if (ctx->engine && !same_tuning(ctx->engine->cfg, t)) {
         (void)hipStreamSynchronize(as_stream(stream));
         engine_destroy(ctx->engine);
@@ -383,6 +385,7 @@ vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n, cons
         cfg.tile_w = t.tile_w;
         cfg.tile_h = t.tile_h;
         cfg.iters_per_graph = t.iters_per_graph;
+        cfg.refill_lanes = t.refill_lanes;
         VZ_TRY(engine_create(ctx, cfg, &ctx->engine));
     }
     RunOptions opt;

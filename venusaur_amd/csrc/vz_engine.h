@@ -87,6 +87,7 @@ struct EngineConfig {
     uint32_t queue_capacity = 1u << 22;
     uint32_t tile_w = 32, tile_h = 2;   // one wave of camera rays = 32x2 pixels
     uint32_t iters_per_graph = 16;
+    uint32_t refill_lanes = 0;          // 0: per-variant default (kernel side)
 };
 
 struct DeviceBlock {

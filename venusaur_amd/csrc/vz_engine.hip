@@ -17,6 +17,12 @@ constexpr int kStepBlk = 512;   // step-kernel workgroup (8 waves)
 constexpr int kStackS = 8;      // LDS stack slots per lane
 constexpr int kTbl = 128;       // LDS (request, pass) tally table entries
 constexpr uint32_t kLdsNodes = 512;  // scenes up to this many nodes are staged in LDS
+// A wave goes back for new rays once fewer than this many of its lanes still
+// trace (1 = only when all are done, 65 = after every node step).  Defaults per
+// node source, chosen by sweeping the 1080p book scene (LDS nodes) and
+// 1M-sphere box scenes (global nodes).
+constexpr uint32_t kRefillLds = 1;
+constexpr uint32_t kRefillGlobal = 41;
 constexpr int kRegenBlk = 256;
 constexpr int kBlk = 256;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
@@ -61,7 +67,13 @@ __device__ __forceinline__ void terminate(const ReqDesc& D, uint32_t slot, uint3
 }
 
 // ---- step: trace + miss/shade stages + compaction, one slice per workgroup ----
-template <bool COUNT, bool LDSN>
+// Persistent traversal with per-lane refill (Aila & Laine, "Understanding the
+// Efficiency of Ray Traversal on GPUs", HPG 2009): a lane whose ray has
+// finished runs the miss or shade stage for it and then pulls the next ray of
+// the slice, so a wave is not held by its slowest ray.  SINGLE: every ray of
+// the launch belongs to scene 0 (the scene descriptor lives in scalars);
+// LDSN: that scene's node array is staged in LDS.
+template <bool COUNT, bool LDSN, bool SINGLE>
 __global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restrict__ scenes,
                                                       const ReqDesc* __restrict__ req,
                                                       const float4* __restrict__ q_o,
@@ -71,7 +83,8 @@ __global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restric
                                                       float4* __restrict__ n_t,
                                                       const uint32_t* __restrict__ cnt_cur,
                                                       uint32_t* __restrict__ cnt_next,
-                                                      BlockStats* __restrict__ bstats, uint32_t R)
+                                                      BlockStats* __restrict__ bstats, uint32_t R,
+                                                      uint32_t refill)
 {
     __shared__ uint32_t s_node[kStackS * kStepBlk];
     __shared__ float s_t[kStackS * kStepBlk];
@@ -100,54 +113,69 @@ __global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restric
     const size_t base_off = (size_t)b * R;
     const uint32_t count = cnt_cur[b];
     uint32_t nv = 0, ns = 0, nh = 0, seg = 0;
+    SceneView sc;
+    if (SINGLE) sc = scenes[0];
+    Traversal<kStackS, COUNT> tr;
+    Overflow ov;
+    bool active = false, tdone = false;
+    size_t qi = 0;
+    uint32_t meta = 0;
     for (;;) {
-        uint32_t c0 = 0;
-        if (lane == 0) c0 = atomicAdd(&s_head, 64u);
-        c0 = bcast(c0);
-        if (c0 >= count) break;
-        const uint32_t local = c0 + lane;
-        const size_t qi = base_off + local;
-        bool valid = local < count;
-        uint32_t meta = kDeadRay, sidx = 0xFFFFFFFFu;
-        if (valid) {
-            meta = __float_as_uint(q_t[qi].w);
-            valid = meta != kDeadRay;
-            if (valid) sidx = req[meta >> 16].scene;
-        }
-        // trace stage: only the ray is live across the traversal; the path
-        // state is re-read afterwards.  A wave whose rays share one scene
-        // (every plain launch) reads the scene descriptor into scalars.
-        TraceResult r{0.0f, 0xFFFFFFFFu};
-        {
-            float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f), d4 = o4;
-            if (valid) {
-                o4 = q_o[qi];
-                d4 = q_d[qi];
-            }
-            const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
-            const uint64_t vm = __ballot(valid);
-            const uint32_t s0 = __shfl(sidx, vm ? __ffsll((unsigned long long)vm) - 1 : 0);
-            if (LDSN) {
-                const SceneView sc = scenes[0];
-                if (valid)
-                    r = closest_hit<kStackS, COUNT>(sc, LdsNodes{s_bvh}, o, d, 0.001f, 1e16f, s_node + tid, s_t + tid,
-                                                    kStepBlk, nv, ns);
-            } else if (__ballot(valid && sidx != s0) == 0ull) {
-                const SceneView sc = scenes[bcast(s0)];
-                if (valid)
-                    r = closest_hit<kStackS, COUNT>(sc, GlobalNodes{sc.nodes}, o, d, 0.001f, 1e16f, s_node + tid,
-                                                    s_t + tid, kStepBlk, nv, ns);
-            } else if (valid) {
-                const SceneView sc = scenes[sidx];
-                r = closest_hit<kStackS, COUNT>(sc, GlobalNodes{sc.nodes}, o, d, 0.001f, 1e16f, s_node + tid,
-                                                s_t + tid, kStepBlk, nv, ns);
+        // refill: idle lanes claim the next rays of the slice, one LDS atomic per wave
+        const bool need = !active;
+        const uint64_t m = __ballot(need);
+        bool exhausted = false;
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&s_head, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            exhausted = base + (uint32_t)__popcll(m) >= count;
+            const uint32_t local = base + mbcnt(m);
+            if (need && local < count) {
+                qi = base_off + local;
+                meta = __float_as_uint(q_t[qi].w);
+                if (meta != kDeadRay) {
+                    const float4 o4 = q_o[qi], d4 = q_d[qi];
+                    if (!SINGLE) sc = scenes[req[meta >> 16].scene];
+                    tdone = tr.init(sc, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), 0.001f, 1e16f, ns);
+                    active = true;
+                }
             }
         }
+        if (__ballot(active) == 0ull) {
+            if (exhausted) break;
+            continue;  // claimed only padding items
+        }
+        // trace stage: advance every busy lane until enough of them finish
+        if (refill <= 1u) {  // the whole wave finishes: a plain divergent loop
+            while (active && !tdone) {
+                if (LDSN)
+                    tdone = tr.step(sc, LdsNodes{s_bvh}, s_node + tid, s_t + tid, kStepBlk, ov, nv, ns);
+                else
+                    tdone = tr.step(sc, GlobalNodes{sc.nodes}, s_node + tid, s_t + tid, kStepBlk, ov, nv, ns);
+            }
+        } else {
+            for (;;) {
+                if (active && !tdone) {
+                    if (LDSN)
+                        tdone = tr.step(sc, LdsNodes{s_bvh}, s_node + tid, s_t + tid, kStepBlk, ov, nv, ns);
+                    else
+                        tdone = tr.step(sc, GlobalNodes{sc.nodes}, s_node + tid, s_t + tid, kStepBlk, ov, nv, ns);
+                }
+                const uint64_t busy = __ballot(active && !tdone);
+                if (busy == 0ull) break;
+                if (!exhausted && (uint32_t)__popcll(busy) < refill) break;
+            }
+        }
+        // miss and shade stages for the lanes whose ray finished
+        const bool fin = active && tdone;
+        if (__ballot(fin) == 0ull) continue;
         bool ended = false, survive = false;
         uint32_t key = 0, pathseg = 0, seed = 0;
         v3 pos = mk(0.0f, 0.0f, 0.0f), nd = pos, thr = pos;
         float4 o4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (valid) {
+        if (fin) {
             o4 = q_o[qi];
             const float4 d4 = q_d[qi], t4 = q_t[qi];
             const uint32_t rq = meta >> 16, slot = (meta >> 8) & 0xFFu, depth = meta & 0xFFu;
@@ -157,18 +185,17 @@ __global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restric
             seed = __float_as_uint(d4.w);
             v3 L = mk(0.0f, 0.0f, 0.0f);
             ended = true;
-            if (r.leaf == 0xFFFFFFFFu) {
+            if (tr.bleaf == 0xFFFFFFFFu) {
                 // miss stage (RayTracer.cu:442-450)
                 L = mul(mk(t4.x, t4.y, t4.z), sky(d));
             } else {
                 nh++;
                 if (depth > 0u) {
                     // shade stage (RayTracer.cu:272-440)
-                    const SceneView sc = scenes[sidx];
-                    const float4 s = sc.spheres[r.leaf];
-                    const uint32_t type = sc.leaf_info[r.leaf] >> 30;
-                    const float4 mat = sc.mats[r.leaf];
-                    pos = add(o, scl(d, r.t));
+                    const float4 s = sc.spheres[tr.bleaf];
+                    const uint32_t type = sc.leaf_info[tr.bleaf] >> 30;
+                    const float4 mat = sc.mats[tr.bleaf];
+                    pos = add(o, scl(d, tr.best));
                     v3 n = scl(sub(pos, mk(s.x, s.y, s.z)), 1.0f / s.w);
                     const bool front = dot(d, n) < 0.0f;
                     if (!front) n = neg(n);
@@ -186,6 +213,7 @@ __global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restric
                 key = (rq << 8) | slot;
                 pathseg = D->max_depth - depth;
             }
+            active = false;
         }
         // survivors: one LDS atomic per wave, slots in lane order
         const uint64_t sm = __ballot(survive);
@@ -557,8 +585,8 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
     Engine* e = new Engine();
     e->ctx = ctx;
     e->cfg = cfg;
-    if (e->cfg.tile_w == 0) e->cfg.tile_w = 8;
-    if (e->cfg.tile_h == 0) e->cfg.tile_h = 8;
+    if (e->cfg.tile_w == 0) e->cfg.tile_w = EngineConfig().tile_w;
+    if (e->cfg.tile_h == 0) e->cfg.tile_h = EngineConfig().tile_h;
     if (e->cfg.iters_per_graph < 2) e->cfg.iters_per_graph = 2;
     e->cfg.iters_per_graph &= ~1u;
     auto fail = [&](vzStatus s, const char* what) {
@@ -567,10 +595,10 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
     };
     int occ = 0;
     int occ_lds = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_wf_step<false, false>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_wf_step<false, false, false>),
                                                      kStepBlk, 0) != hipSuccess || occ < 1)
         occ = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_lds, reinterpret_cast<const void*>(k_wf_step<false, true>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_lds, reinterpret_cast<const void*>(k_wf_step<false, true, true>),
                                                      kStepBlk, 0) == hipSuccess && occ_lds >= 1)
         occ = std::min(occ, occ_lds);
     uint32_t cus = ctx->num_cus ? ctx->num_cus : 256;
@@ -616,20 +644,23 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
 void engine_destroy(Engine* e) { delete e; }
 
 // Enqueues one wavefront iteration of parity p (reads slices p, writes 1-p).
-static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool lds, hipEvent_t ev0,
+static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool lds, bool single, hipEvent_t ev0,
                               hipEvent_t ev1)
 {
     const int n = 1 - p;
     uint32_t* cnt_cur = e->d_bcnt + (size_t)p * e->NB;
     uint32_t* cnt_nxt = e->d_bcnt + (size_t)n * e->NB;
     if (ev0) (void)hipEventRecord(ev0, st);
-#define VZ_STEP(C, L)                                                                                      \
-    hipLaunchKernelGGL((k_wf_step<C, L>), dim3(e->NB), dim3(kStepBlk), 0, st, e->d_scenes, e->d_req, e->q_o[p], \
-                       e->q_d[p], e->q_t[p], e->q_o[n], e->q_d[n], e->q_t[n], cnt_cur, cnt_nxt, e->d_bstats, e->R)
+#define VZ_STEP(C, L, S1)                                                                                  \
+    hipLaunchKernelGGL((k_wf_step<C, L, S1>), dim3(e->NB), dim3(kStepBlk), 0, st, e->d_scenes, e->d_req,        \
+                       e->q_o[p], e->q_d[p], e->q_t[p], e->q_o[n], e->q_d[n], e->q_t[n], cnt_cur, cnt_nxt,      \
+                       e->d_bstats, e->R, refill)
+    const uint32_t refill = e->cfg.refill_lanes ? std::min<uint32_t>(e->cfg.refill_lanes, 65u)
+                                                : (lds ? kRefillLds : kRefillGlobal);
     if (count) {
-        if (lds) VZ_STEP(true, true); else VZ_STEP(true, false);
+        if (lds) VZ_STEP(true, true, true); else if (single) VZ_STEP(true, false, true); else VZ_STEP(true, false, false);
     } else {
-        if (lds) VZ_STEP(false, true); else VZ_STEP(false, false);
+        if (lds) VZ_STEP(false, true, true); else if (single) VZ_STEP(false, false, true); else VZ_STEP(false, false, false);
     }
 #undef VZ_STEP
     if (ev1) (void)hipEventRecord(ev1, st);
@@ -640,9 +671,9 @@ static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool
                        e->q_d[n], e->q_t[n], e->d_bstats, e->R);
 }
 
-static vzStatus ensure_graph(Engine* e, bool count, bool lds)
+static vzStatus ensure_graph(Engine* e, bool count, bool lds, bool single)
 {
-    const int key = (count ? 1 : 0) | (lds ? 2 : 0);
+    const int key = (count ? 1 : 0) | (lds ? 2 : 0) | (single ? 4 : 0);
     if (e->graph && e->graph_key == key) return VZ_SUCCESS;
     if (e->graph) {
         (void)hipGraphExecDestroy(e->graph);
@@ -651,7 +682,7 @@ static vzStatus ensure_graph(Engine* e, bool count, bool lds)
     if (!e->graph_stream) VZ_HIP_TRY(hipStreamCreateWithFlags(&e->graph_stream, hipStreamNonBlocking));
     VZ_HIP_TRY(hipStreamBeginCapture(e->graph_stream, hipStreamCaptureModeThreadLocal));
     for (uint32_t it = 0; it < e->cfg.iters_per_graph; it++)
-        enqueue_iteration(e, e->graph_stream, (int)(it & 1u), count, lds, nullptr, nullptr);
+        enqueue_iteration(e, e->graph_stream, (int)(it & 1u), count, lds, single, nullptr, nullptr);
     hipGraph_t g = nullptr;
     VZ_HIP_TRY(hipStreamEndCapture(e->graph_stream, &g));
     hipError_t err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
@@ -782,8 +813,9 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
 
     const bool use_graph = opt.use_graph && !opt.time_trace;
     // a single small scene is staged in LDS by every step workgroup
-    const bool lds = scenes.size() == 1 && scenes[0]->n >= 2 && scenes[0]->n - 1 <= kLdsNodes;
-    if (use_graph) VZ_TRY(ensure_graph(e, opt.count_visits, lds));
+    const bool single = scenes.size() == 1;
+    const bool lds = single && scenes[0]->n >= 2 && scenes[0]->n - 1 <= kLdsNodes;
+    if (use_graph) VZ_TRY(ensure_graph(e, opt.count_visits, lds, single));
     const uint32_t B = e->cfg.iters_per_graph;
     hipEvent_t t0, t1, blk_ev[2];
     VZ_HIP_TRY(hipEventCreate(&t0));
@@ -806,7 +838,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
             for (uint32_t it = 0; it < B; it++) {
                 hipEvent_t a = opt.time_trace ? tev[k & 1][2 * it] : nullptr;
                 hipEvent_t b = opt.time_trace ? tev[k & 1][2 * it + 1] : nullptr;
-                enqueue_iteration(e, st, (int)(it & 1u), opt.count_visits, lds, a, b);
+                enqueue_iteration(e, st, (int)(it & 1u), opt.count_visits, lds, single, a, b);
             }
             VZ_HIP_TRY(hipGetLastError());
         }
@@ -913,7 +945,7 @@ vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes)
     VZ_HIP_TRY(hipMemsetAsync(e->d_cnt, 0, sizeof(Counters), st));
     VZ_HIP_TRY(hipMemsetAsync(e->d_bcnt, 0, 2 * e->NB * sizeof(uint32_t), st));
     VZ_HIP_TRY(hipMemsetAsync(e->d_bstats, 0, e->NB * sizeof(BlockStats), st));
-    VZ_TRY(ensure_graph(e, false, false));
+    VZ_TRY(ensure_graph(e, false, false, false));
     VZ_HIP_TRY(hipStreamSynchronize(st));
     return VZ_SUCCESS;
 }

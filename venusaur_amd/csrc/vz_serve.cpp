@@ -213,6 +213,7 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     if (s->cfg.tuning.tile_w) ec.tile_w = s->cfg.tuning.tile_w;
     if (s->cfg.tuning.tile_h) ec.tile_h = s->cfg.tuning.tile_h;
     if (s->cfg.tuning.iters_per_graph >= 2) ec.iters_per_graph = s->cfg.tuning.iters_per_graph;
+    ec.refill_lanes = std::min<uint32_t>(s->cfg.tuning.refill_lanes, 65u);
     vzStatus rs = engine_create(ctx, ec, &s->engine);
     if (rs == VZ_SUCCESS && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         rs = set_error(VZ_ERROR_HIP, "hipStreamCreateWithFlags failed");
@@ -314,7 +315,7 @@ vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStrea
             cand.tile_w = t[0];
             cand.tile_h = t[1];
             cand.iters_per_graph = 16;
-            cand.stack_lds = 8;
+            cand.refill_lanes = 0;
             vzStats st{};
             // first run warms the engine allocation and graph, second is timed
             VZ_TRY(vzBatchLaunch(ctx, probe.data(), n, &cand, stream, &st));

@@ -43,48 +43,77 @@ struct LdsNodes {
     }
 };
 
-// S: LDS slots per lane (power of two).  lds_node/lds_t point at this lane's
-// column; consecutive slots are `stride` words apart.
-template <int S, bool COUNT, class NodeSrc>
-VZ_DEV TraceResult closest_hit(const SceneView& sc, const NodeSrc& nodes, v3 o, v3 d, float tmin,
-                               float tmax, uint32_t* lds_node, float* lds_t, uint32_t stride,
-                               uint32_t& nv, uint32_t& ns)
-{
-    float best = tmax;
-    uint32_t bleaf = 0xFFFFFFFFu, bprim = 0xFFFFFFFFu;
-    if (sc.n == 1u) {
-        float t;
-        if (COUNT) ns++;
-        if (hit_sphere(o, d, tmin, best, sc.spheres[0], t)) {
-            best = t;
-            bleaf = 0;
+// The oldest stack entries of a lane once its LDS ring is full.  Kept apart
+// from Traversal so the traversal's scalars stay in registers: the compiler
+// only splits an aggregate into registers when nothing in it is indexed
+// dynamically.
+struct Overflow {
+    uint32_t node[kMaxDepth];
+    float t[kMaxDepth];
+};
+
+// Resumable traversal of one ray: init() sets it up, every step() visits
+// one internal node.  A wave can therefore refill a lane with a new ray as
+// soon as its traversal ends instead of waiting for the slowest lane.
+// S: LDS slots per lane (power of two); lds_node/lds_t point at this lane's
+// column, consecutive slots `stride` words apart.
+template <int S, bool COUNT>
+struct Traversal {
+    v3 o, d;
+    float inv[3];
+    float tmin, best;
+    uint32_t bleaf, bprim, node, sp, bottom, budget;
+
+    // Returns true when the ray is already finished (single-sphere scene).
+    VZ_DEV bool init(const SceneView& sc, v3 o_, v3 d_, float tmin_, float tmax_, uint32_t& ns)
+    {
+        o = o_;
+        d = d_;
+        tmin = tmin_;
+        best = tmax_;
+        bleaf = 0xFFFFFFFFu;
+        bprim = 0xFFFFFFFFu;
+        if (sc.n == 1u) {
+            float t;
+            if (COUNT) ns++;
+            if (hit_sphere(o, d, tmin, best, sc.spheres[0], t)) {
+                best = t;
+                bleaf = 0;
+            }
+            return true;
         }
-        return TraceResult{best, bleaf};
-    }
-    float inv[3], org[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+        const float dd[3] = {d.x, d.y, d.z};
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        float dk = dd[k];
-        if (!(fabsf(dk) > 1e-20f)) dk = copysignf(1e-20f, dk);
-        inv[k] = 1.0f / dk;
+        for (int k = 0; k < 3; k++) {
+            float dk = dd[k];
+            if (!(fabsf(dk) > 1e-20f)) dk = copysignf(1e-20f, dk);
+            inv[k] = 1.0f / dk;
+        }
+        sp = 0;
+        bottom = 0;
+        node = 0;
+        // a well-formed tree visits each of its n-1 internal nodes at most
+        // once; this and the index guards below only matter for a corrupted
+        // node array
+        budget = sc.n;
+        return false;
     }
-    uint32_t ov_node[kMaxDepth];
-    float ov_t[kMaxDepth];
-    uint32_t sp = 0, bottom = 0;
-    uint32_t node = 0;
-    // a well-formed tree visits each of its n-1 internal nodes at most once;
-    // the guards below only matter for a corrupted node array
-    uint32_t budget = sc.n;
-    for (;;) {
-        if (budget-- == 0u) break;
+
+    // Visits one node; returns true when the traversal has finished.
+    template <class NodeSrc>
+    VZ_DEV bool step(const SceneView& sc, const NodeSrc& nodes, uint32_t* lds_node, float* lds_t,
+                     uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns)
+    {
+        if (budget-- == 0u) return true;
         uint4 A, B;
         nodes.fetch(node, A, B);
         if (COUNT) nv++;
         const float no[3] = {__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z)};
+        const float org[3] = {o.x, o.y, o.z};
         const uint32_t meta = A.w;
         const uint32_t qw[3] = {B.x, B.y, B.z};
         const uint32_t g = B.w;
-        if (g + 1u >= sc.n) break;
+        if (g + 1u >= sc.n) return true;
         float tn[2];
         bool hit[2];
 #pragma unroll
@@ -130,37 +159,36 @@ VZ_DEV TraceResult closest_hit(const SceneView& sc, const NodeSrc& nodes, v3 o, 
         }
         const bool in0 = hit[0] && !(flags & 1u);
         const bool in1 = hit[1] && !(flags & 2u);
-        if (in1 && g + 2u >= sc.n) break;  // internal child g+1 must be <= n-2
+        if (in1 && g + 2u >= sc.n) return true;  // internal child g+1 must be <= n-2
         if (in0 && in1) {
             const int nr = (tn[1] < tn[0]) ? 1 : 0;
             const uint32_t far_node = g + (uint32_t)(1 - nr);
             const float far_t = tn[1 - nr];
             if (sp - bottom == (uint32_t)S) {  // ring full: oldest entry to the overflow
-                if (bottom >= kMaxDepth) break;
-                ov_node[bottom] = lds_node[(bottom & (S - 1)) * stride];
-                ov_t[bottom] = lds_t[(bottom & (S - 1)) * stride];
+                if (bottom >= kMaxDepth) return true;
+                ov.node[bottom] = lds_node[(bottom & (S - 1)) * stride];
+                ov.t[bottom] = lds_t[(bottom & (S - 1)) * stride];
                 bottom++;
             }
             lds_node[(sp & (S - 1)) * stride] = far_node;
             lds_t[(sp & (S - 1)) * stride] = far_t;
             sp++;
             node = g + (uint32_t)nr;
-            continue;
+            return false;
         } else if (in0) {
             node = g;
-            continue;
+            return false;
         } else if (in1) {
             node = g + 1u;
-            continue;
+            return false;
         }
-        bool found = false;
-        while (sp > 0u) {
+        while (sp > 0u) {  // pop, culling entries beyond the best hit
             sp--;
             uint32_t nd;
             float t;
             if (sp < bottom) {
-                nd = ov_node[sp];
-                t = ov_t[sp];
+                nd = ov.node[sp];
+                t = ov.t[sp];
                 bottom = sp;
             } else {
                 nd = lds_node[(sp & (S - 1)) * stride];
@@ -168,13 +196,25 @@ VZ_DEV TraceResult closest_hit(const SceneView& sc, const NodeSrc& nodes, v3 o, 
             }
             if (t <= best * kRobust) {
                 node = nd;
-                found = true;
-                break;
+                return false;
             }
         }
-        if (!found) break;
+        return true;
     }
-    return TraceResult{best, bleaf};
+};
+
+// Whole-ray traversal: init, then step until done.
+template <int S, bool COUNT, class NodeSrc>
+VZ_DEV TraceResult closest_hit(const SceneView& sc, const NodeSrc& nodes, v3 o, v3 d, float tmin,
+                               float tmax, uint32_t* lds_node, float* lds_t, uint32_t stride,
+                               uint32_t& nv, uint32_t& ns)
+{
+    Traversal<S, COUNT> tr;
+    Overflow ov;
+    if (!tr.init(sc, o, d, tmin, tmax, ns))
+        while (!tr.step(sc, nodes, lds_node, lds_t, stride, ov, nv, ns)) {
+        }
+    return TraceResult{tr.best, tr.bleaf};
 }
 
 }  // namespace vz
