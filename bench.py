@@ -45,8 +45,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (scales rocprof FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--batch-scenes", type=int, default=16, help="scenes in the batched-form comparison (0 = skip)")
+    ap.add_argument("--batch-scenes", type=int, default=64, help="scenes in the batched-form comparison (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=120, help="requests in the serving comparison (0 = skip)")
+    ap.add_argument("--c2-frames", type=int, default=1,
+                    help="timed frames of the 1M-sphere traversal-bound scene (0 = skip)")
     return ap.parse_args()
 
 
@@ -66,7 +68,23 @@ def _build_scenes(ctx, reqs):
     return scenes, build_ms
 
 
-def run_batched(ctx, n_scenes):
+def _rmse_fp64(r):
+    """RMSE of a bound request's GPU accumulator against the FP64 oracle."""
+    import oracle as O
+    s, m = O.box_scene(r.n_small, 1000 + r.scene_key)
+    cam = O.camera_lookat(list(r.lookfrom), list(r.lookat), r.vfov, float(np.float32(r.width) / np.float32(r.height)),
+                          0.0, 10.0)
+    p = O.make_params(r.width, r.height, r.spp, r.depth, cam)
+    acc64, _ = O.render_f64(s, m, p, bvh=O.BVH(s))
+    d = r.accum_host()[..., :3].astype(np.float64) - acc64[..., :3]
+    return float(np.sqrt(np.mean(d * d)))
+
+
+def _cheapest(reqs, k):
+    return sorted(range(len(reqs)), key=lambda i: (reqs[i].paths * reqs[i].n_small))[:k]
+
+
+def run_batched(ctx, n_scenes, rmse_samples=2):
     """Batched form vs sequential plain launches of the same 64-scene mix."""
     from venusaur_amd import api, workloads
     from venusaur_amd.hipmem import synchronize
@@ -87,14 +105,77 @@ def run_batched(ctx, n_scenes):
     st = api.batch_launch(ctx, items, tuning=tuning)
     t_batch = time.perf_counter() - t0
     same = all(_sha(r.accum_host()) == w for r, w in zip(reqs, want))
+    rmse64 = {f"item{k}_{reqs[k].width}px_{reqs[k].spp}spp_{reqs[k].n_small}sph": _rmse_fp64(reqs[k])
+              for k in _cheapest(reqs, rmse_samples)}
     out = {"workload": f"{n_scenes} box scenes (1e3-2e5 spheres), 256-1024 px square, 4/16/64 spp, depth 16",
            "segments": int(st.segments), "plain_sequential_mseg_s": round(segs_plain / t_plain / 1e6, 1),
            "batched_mseg_s": round(st.segments / t_batch / 1e6, 1),
            "speedup_vs_plain": round(t_plain / t_batch, 2), "plain_s": round(t_plain, 3),
            "batched_s": round(t_batch, 3), "bit_identical_to_plain": same, "segments_match": segs_plain == st.segments,
-           "tuning": tuning.as_dict(), "lbvh_build_ms_mean": round(float(np.mean(build_ms)), 3)}
+           "tuning": tuning.as_dict(), "lbvh_build_ms_mean": round(float(np.mean(build_ms)), 3),
+           "rmse_vs_fp64_oracle": rmse64}
     for sc in scenes.values():
         sc.close()
+    return out
+
+
+def step_bytes(st):
+    """Algorithmic bytes of the step kernel (trace + miss/shade stages) for a
+    counting launch: per segment the 48 B path record read, 32 B per BVH node
+    fetched, 16 B per sphere tested; per hit 36 B of sphere, material and type
+    re-read by the shade stage; per surviving path the 48 B record written to
+    the next queue; per ended path its 12 B radiance written to the pass
+    buffer."""
+    return (48 * st.segments + 32 * st.node_visits + 16 * st.sphere_tests + 36 * st.hits
+            + 48 * (st.segments - st.paths) + 12 * st.paths)
+
+
+def run_c2(ctx, frames, seed=2024):
+    """Traversal-bound plain method: 1M box spheres, 1080p, 64 spp, depth 16."""
+    from venusaur_amd import api
+    from venusaur_amd.hipmem import DeviceBuffer, synchronize
+    W, H, SPP, DEPTH, N = 1920, 1080, 64, 16, 1_000_000
+    t0 = time.perf_counter()
+    s, m = api.box_scene(N, seed)
+    t_gen = time.perf_counter() - t0
+    sc = api.Scene(ctx, s, m)
+    info = sc.info()
+    p = api.launch_params(W, H, SPP, DEPTH, [0, 15, -120], [0, 0, 0], 40.0, 0.0, 10.0,
+                          aspect=float(np.float32(W) / np.float32(H)))
+    acc, img = DeviceBuffer(W * H * 16), DeviceBuffer(W * H * 4)
+    p.accum, p.image = acc.value, img.value
+    sc.launch(p)  # warmup
+    pc = api.LaunchParams.from_buffer_copy(p)
+    pc.flags = api.COUNT_VISITS
+    cst = sc.launch(pc)
+    pt = api.LaunchParams.from_buffer_copy(p)
+    pt.flags = api.TIME_STAGES
+    synchronize()
+    t0 = time.perf_counter()
+    stats = [sc.launch(pt) for _ in range(frames)]
+    synchronize()
+    wall = time.perf_counter() - t0
+    segs = sum(x.segments for x in stats)
+    launches = sum(x.trace_launches for x in stats)
+    avg_ms = sum(x.trace_ms for x in stats) / launches
+    per_launch = step_bytes(cst) * frames / launches
+    out = {"workload": f"{N} box spheres + ground (seed {seed}), {W}x{H} {SPP}spp depth {DEPTH} plain (wavefront)",
+           "frames": frames, "value": round(segs / wall / 1e6, 2), "unit": "Mray-segments/s",
+           "ms_per_frame": round(wall / frames * 1e3, 2), "segments_per_frame": int(stats[0].segments),
+           "lbvh_build_ms": round(info.build_ms, 3), "bvh_depth": int(info.depth),
+           "host_generate_s": round(t_gen, 3), "node_visits_per_segment": round(cst.node_visits / cst.segments, 2),
+           "sphere_tests_per_segment": round(cst.sphere_tests / cst.segments, 2),
+           "step_kernel": {"launches": launches, "avg_launch_ms": round(avg_ms, 4),
+                           "algorithmic_bytes_per_launch": round(per_launch),
+                           "algorithmic_gbs": round(per_launch / (avg_ms * 1e-3) / 1e9, 1)}}
+    tf = os.path.join(ROOT, "profiles", "step_kernel_traffic_c2.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("traffic_bytes_per_launch")
+        out["step_kernel"]["traffic"] = traffic
+        out["step_kernel"]["measured_hbm_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+    acc.free()
+    img.free()
+    sc.close()
     return out
 
 
@@ -103,7 +184,6 @@ def run_served(ctx, n_requests, rmse_samples=3):
     import threading
     from venusaur_amd import api, workloads
     from venusaur_amd.hipmem import synchronize
-    import oracle as O
     # calibrate the plain capacity on the stream's own mix, then overload it 3x
     probe = workloads.request_stream(24, 1.0, seed=12)
     pscenes, _ = _build_scenes(ctx, probe)
@@ -156,19 +236,9 @@ def run_served(ctx, n_requests, rmse_samples=3):
     same = all(_sha(r.accum_host()) == w for r, w in zip(reqs, want))
     ok = all(x.status == 0 for x in res)
     srv.close()
-    # RMSE against the FP64 oracle on the smallest requests
-    small = sorted(range(len(reqs)), key=lambda k: (reqs[k].paths * reqs[k].n_small))[:rmse_samples]
-    rmse64 = {}
-    for k in small:
-        r = reqs[k]
-        s, m = O.box_scene(r.n_small, 1000 + r.scene_key)
-        cam = O.camera_lookat(list(r.lookfrom), list(r.lookat), r.vfov, float(np.float32(r.width) / np.float32(r.height)),
-                              0.0, 10.0)
-        p = O.make_params(r.width, r.height, r.spp, r.depth, cam)
-        acc64, _ = O.render_f64(s, m, p, bvh=O.BVH(s))
-        got = r.accum_host()
-        d = got[..., :3].astype(np.float64) - acc64[..., :3]
-        rmse64[f"req{k}_{r.width}px_{r.spp}spp_{r.n_small}sph"] = float(np.sqrt(np.mean(d * d)))
+    # RMSE against the FP64 oracle on the cheapest requests
+    rmse64 = {f"req{k}_{reqs[k].width}px_{reqs[k].spp}spp_{reqs[k].n_small}sph": _rmse_fp64(reqs[k])
+              for k in _cheapest(reqs, rmse_samples)}
     for sc in scenes.values():
         sc.close()
     pct = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 1)  # noqa: E731
@@ -236,13 +306,9 @@ def main():
     gpu_acc = acc.download(np.float32, (H, W, 4))
 
     # roofline of the step kernel (trace + miss/shade stages, the dominant
-    # kernel).  Algorithmic bytes: per segment the 48 B path record read, 32 B
-    # per BVH node fetched, 16 B per sphere tested; per hit 36 B of sphere,
-    # material and type re-read by the shade stage; per surviving path the
-    # 48 B record written to the next queue; per ended path its 12 B
-    # radiance written to the pass buffer.
-    seg_c, nv_c, ns_c, hit_c, path_c = cst.segments, cst.node_visits, cst.sphere_tests, cst.hits, cst.paths
-    bytes_step = (48 * seg_c + 32 * nv_c + 16 * ns_c + 36 * hit_c + 48 * (seg_c - path_c) + 12 * path_c)
+    # kernel), from the algorithmic bytes of the counting launch
+    seg_c, nv_c, ns_c = cst.segments, cst.node_visits, cst.sphere_tests
+    bytes_step = step_bytes(cst)
     trace_ms = sum(s.trace_ms for s in stats)
     launches = sum(s.trace_launches for s in stats)
     roof = None
@@ -302,6 +368,8 @@ def main():
         checks["fp64_sample"] = f"rows y%{r64}==0"
 
     forms = {}
+    if a.c2_frames:
+        forms["traversal_bound"] = run_c2(ctx, a.c2_frames)
     if a.batch_scenes:
         forms["batched"] = run_batched(ctx, a.batch_scenes)
     if a.serve_requests:
