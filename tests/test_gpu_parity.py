@@ -48,7 +48,7 @@ def test_lbvh_book_bit_exact(ctx, book):
     compare_bvh(sc, s)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 2047, 2048, 2049, 20000, 300000])
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 2047, 2048, 2049, 4095, 4097, 20000, 300000, 1048583])
 def test_lbvh_sizes_bit_exact(ctx, n):
     say(f"case: lbvh box scene n={n}")
     s, m = api.box_scene(n - 1, 1234 + n) if n > 1 else api.box_scene(0, 1)

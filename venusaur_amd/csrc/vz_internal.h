@@ -84,6 +84,8 @@ struct vzScene_st {
     uint32_t* d_parent_leaf = nullptr;
     uint32_t* d_visit = nullptr;
     uint32_t* d_height = nullptr;
+    uint32_t* d_frontier = nullptr;   // refit: finished children of workgroup-crossing nodes
+    uint32_t* d_fcount = nullptr;
     uint32_t depth = 0;
     uint64_t device_bytes = 0;
     double build_ms = 0.0;

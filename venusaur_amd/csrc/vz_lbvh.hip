@@ -28,6 +28,8 @@ namespace {
 constexpr int kBlk = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBlk * kItems;  // keys per radix tile
+constexpr uint32_t kLocalBit = 0x80000000u;  // parent link flag (see k_karras)
+constexpr uint32_t kRefitLeaves = 2048;       // leaves per k_refit_local workgroup
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
@@ -60,20 +62,33 @@ __global__ void __launch_bounds__(kBlk) k_bounds_partial(const float4* __restric
     if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// out: lo[3], hi[3], scale[3] with scale = ext > 0 ? 1024/ext : 0
-__global__ void k_bounds_final(const float* __restrict__ partial, uint32_t nb, float* __restrict__ out)
+// out: lo[3], hi[3], scale[3] with scale = ext > 0 ? 1024/ext : 0.  min and
+// max are exact, so the reduction order does not change the result.
+__global__ void __launch_bounds__(kBlk) k_bounds_final(const float* __restrict__ partial, uint32_t nb,
+                                                       float* __restrict__ out)
 {
-    if (threadIdx.x != 0) return;
-    float lo[3] = {partial[0], partial[1], partial[2]}, hi[3] = {partial[3], partial[4], partial[5]};
-    for (uint32_t b = 1; b < nb; b++)
+    __shared__ float s[6][kBlk];
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (uint32_t b = threadIdx.x; b < nb; b += kBlk)
         for (int k = 0; k < 3; k++) {
             lo[k] = fminf(lo[k], partial[b * 6 + k]);
             hi[k] = fmaxf(hi[k], partial[b * 6 + 3 + k]);
         }
-    for (int k = 0; k < 3; k++) {
-        float ext = hi[k] - lo[k];
-        out[k] = lo[k];
-        out[3 + k] = hi[k];
+    for (int k = 0; k < 3; k++) { s[k][threadIdx.x] = lo[k]; s[3 + k][threadIdx.x] = hi[k]; }
+    __syncthreads();
+    for (int w = kBlk / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        const int k = (int)threadIdx.x;
+        const float l = s[k][0], h = s[3 + k][0], ext = h - l;
+        out[k] = l;
+        out[3 + k] = h;
         out[6 + k] = ext > 0.0f ? 1024.0f / ext : 0.0f;
     }
 }
@@ -108,6 +123,22 @@ __global__ void __launch_bounds__(kBlk) k_morton(const float4* __restrict__ sph,
 }
 
 // ---- 3. radix sort -------------------------------------------------------------
+// Exclusive scan of one value per thread over a kBlk-thread workgroup.
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* s)
+{
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < (uint32_t)kBlk; off <<= 1) {
+        const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const uint32_t r = s[threadIdx.x] - v;
+    __syncthreads();
+    return r;
+}
+
 __global__ void __launch_bounds__(kBlk) k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                      int shift, uint32_t nblocks,
                                                      uint32_t* __restrict__ hist)
@@ -124,37 +155,37 @@ __global__ void __launch_bounds__(kBlk) k_radix_hist(const uint32_t* __restrict_
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of `total` words in place, one block of 1024 threads.
-__global__ void __launch_bounds__(1024) k_scan_excl(uint32_t* __restrict__ a, uint32_t total)
+// Per digit (one workgroup each): exclusive scan in place of the digit's
+// per-tile counts, row total to dtot[digit].  The scatter adds the digit's
+// base, the exclusive scan of dtot.
+__global__ void __launch_bounds__(kBlk) k_scan_rows(uint32_t* __restrict__ hist, uint32_t nblocks,
+                                                    uint32_t* __restrict__ dtot)
 {
-    __shared__ uint32_t part[1024];
-    uint32_t chunk = (total + 1023) / 1024;
-    uint32_t b = threadIdx.x * chunk, e = min(b + chunk, total);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; i++) s += a[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;
+    __shared__ uint32_t part[kBlk];
+    uint32_t* row = hist + (size_t)blockIdx.x * nblocks;
+    const uint32_t chunk = (nblocks + kBlk - 1) / kBlk;
+    const uint32_t b = threadIdx.x * chunk, e = min(b + chunk, nblocks);
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; i++) sum += row[i];
+    const uint32_t excl = block_scan_excl(sum, part);
+    uint32_t run = excl;
     for (uint32_t i = b; i < e; i++) {
-        uint32_t v = a[i];
-        a[i] = run;
+        const uint32_t v = row[i];
+        row[i] = run;
         run += v;
     }
+    if (threadIdx.x == kBlk - 1) dtot[blockIdx.x] = excl + sum;
 }
 
 __global__ void __launch_bounds__(kBlk) k_radix_scatter(const uint32_t* __restrict__ kin,
                                                         const uint32_t* __restrict__ vin, uint32_t n,
                                                         int shift, uint32_t nblocks,
                                                         const uint32_t* __restrict__ hist,
+                                                        const uint32_t* __restrict__ dtot,
                                                         uint32_t* __restrict__ kout,
                                                         uint32_t* __restrict__ vout)
 {
+    __shared__ uint32_t s_dbase[256];   // first output slot of each digit
     __shared__ uint32_t s_count[256];   // running per-digit count in this tile
     __shared__ uint32_t s_start[256];   // tile-local start of each digit
     __shared__ uint32_t s_wcnt[4][256];
@@ -162,6 +193,7 @@ __global__ void __launch_bounds__(kBlk) k_radix_scatter(const uint32_t* __restri
     __shared__ uint32_t s_key[kTile];
     __shared__ uint32_t s_val[kTile];
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+    s_dbase[tid] = block_scan_excl(dtot[tid], s_count);  // s_count as scratch
     s_count[tid] = 0;
     for (int w = 0; w < 4; w++) s_wcnt[w][tid] = 0;
     __syncthreads();
@@ -225,7 +257,7 @@ __global__ void __launch_bounds__(kBlk) k_radix_scatter(const uint32_t* __restri
     for (uint32_t i = tid; i < cnt; i += kBlk) {
         uint32_t k = s_key[i];
         uint32_t d = (k >> shift) & 0xFFu;
-        uint32_t g = hist[d * nblocks + blockIdx.x] + (i - s_start[d]);
+        uint32_t g = s_dbase[d] + hist[d * nblocks + blockIdx.x] + (i - s_start[d]);
         kout[g] = k;
         vout[g] = s_val[i];
     }
@@ -298,8 +330,11 @@ __global__ void __launch_bounds__(kBlk) k_karras(const uint32_t* __restrict__ K,
     int g = i + s * d + (d < 0 ? d : 0);
     int mn = min(i, j), mx = max(i, j);
     uint32_t f = 0;
-    if (mn == g) { f |= 1u; parent_leaf[g] = (uint32_t)i; } else { parent_int[g] = (uint32_t)i; }
-    if (mx == g + 1) { f |= 2u; parent_leaf[g + 1] = (uint32_t)i; } else { parent_int[g + 1] = (uint32_t)i; }
+    // bit 31 of a parent link: every leaf of the parent's range lies in one
+    // refit workgroup, so the refit synchronises on it at workgroup scope
+    const uint32_t up = (uint32_t)i | ((mn / (int)kRefitLeaves == mx / (int)kRefitLeaves) ? kLocalBit : 0u);
+    if (mn == g) { f |= 1u; parent_leaf[g] = up; } else { parent_int[g] = up; }
+    if (mx == g + 1) { f |= 2u; parent_leaf[g + 1] = up; } else { parent_int[g + 1] = up; }
     child[i] = (uint32_t)g | (f << 30);
 }
 
@@ -335,71 +370,171 @@ __device__ __forceinline__ uint32_t quant_hi(float v, float lo, float s)
     return (uint32_t)q;
 }
 
-__device__ __forceinline__ float ld_relaxed(const float* p)
+
+__device__ __forceinline__ void leaf_box(const float4 sp, float* b)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t ld_relaxed_u(const uint32_t* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const float r = fabsf(sp.w);
+    b[0] = sp.x - r; b[1] = sp.y - r; b[2] = sp.z - r;
+    b[3] = sp.x + r; b[4] = sp.y + r; b[5] = sp.z + r;
 }
 
-__global__ void __launch_bounds__(kBlk) k_refit(const float4* __restrict__ sph, uint32_t n,
-                                                const uint32_t* __restrict__ child,
-                                                const uint32_t* __restrict__ parent_int,
-                                                const uint32_t* __restrict__ parent_leaf,
-                                                uint32_t* visit, float* boxes, uint32_t* height,
-                                                uint4* __restrict__ nodes)
+// Union of the two child boxes into nb, and the 32-byte packed node p.
+__device__ __forceinline__ void pack_node(uint32_t p, const float (&cb)[2][6], uint32_t g, uint32_t flags,
+                                          float (&nb)[6], uint4* __restrict__ nodes)
 {
-    uint32_t leaf = blockIdx.x * kBlk + threadIdx.x;
-    if (leaf >= n) return;
-    uint32_t p = parent_leaf[leaf];
-    for (;;) {
-        __threadfence();
-        uint32_t old = __hip_atomic_fetch_add(&visit[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) return;
-        __threadfence();
-        uint32_t cw = child[p];
-        uint32_t g = cw & 0x3FFFFFFFu, flags = cw >> 30;
-        float cb[2][6];
-        uint32_t h = 0;
-        for (int c = 0; c < 2; c++) {
-            uint32_t ch = g + (uint32_t)c;
-            if ((flags >> c) & 1u) {
-                float4 s = sph[ch];
-                float r = fabsf(s.w);
-                cb[c][0] = s.x - r; cb[c][1] = s.y - r; cb[c][2] = s.z - r;
-                cb[c][3] = s.x + r; cb[c][4] = s.y + r; cb[c][5] = s.z + r;
-            } else {
-                for (int k = 0; k < 6; k++) cb[c][k] = ld_relaxed(&boxes[ch * 6 + k]);
-                h = max(h, ld_relaxed_u(&height[ch]));
-            }
+    for (int k = 0; k < 3; k++) {
+        nb[k] = fminf(cb[0][k], cb[1][k]);
+        nb[3 + k] = fmaxf(cb[0][3 + k], cb[1][3 + k]);
+    }
+    uint32_t meta = 0, q[3];
+    float o[3];
+    for (int k = 0; k < 3; k++) {
+        const float lo = nb[k], hi = nb[3 + k];
+        const uint32_t e = pick_exp(lo, hi);
+        const float sc = __uint_as_float(e << 23);
+        meta |= e << (8 * k);
+        const uint32_t l0 = quant_lo(cb[0][k], lo, sc), h0 = quant_hi(cb[0][3 + k], lo, sc);
+        const uint32_t l1 = quant_lo(cb[1][k], lo, sc), h1 = quant_hi(cb[1][3 + k], lo, sc);
+        o[k] = lo;
+        q[k] = l0 | (h0 << 8) | (l1 << 16) | (h1 << 24);
+    }
+    meta |= flags << 24;
+    nodes[2 * p] = make_uint4(__float_as_uint(o[0]), __float_as_uint(o[1]), __float_as_uint(o[2]), meta);
+    nodes[2 * p + 1] = make_uint4(q[0], q[1], q[2], g);
+}
+
+// Bottom-up refit (Karras 2012, section 4): one thread per leaf climbs, and
+// the second thread to reach a node refits it.  Split in two kernels so that
+// the bulk of the nodes needs no cross-XCD coherence:
+//   k_refit_local  one workgroup per kRefitLeaves leaves.  Every node whose
+//                  leaf range lies inside the workgroup's range also has its
+//                  index there, so its child word, parent link, arrival
+//                  counter, box and height all live in LDS.  A climb stops
+//                  below the first node whose range crosses workgroups and
+//                  publishes the finished child (a leaf or a local node) on
+//                  the frontier list.
+//   k_refit_top    climbs from every frontier entry through the crossing
+//                  nodes (about n / kRefitLeaves * log2 kRefitLeaves of them)
+//                  in global memory at agent scope.  Starting from a clean L2
+//                  (the kernel boundary wrote the local results back), each
+//                  release only writes back the few lines this kernel
+//                  dirtied: 0.8 ms for the whole 1M-sphere build against
+//                  1.2 ms with the top done by one workgroup at workgroup
+//                  scope, and 10 ms with every node at agent scope in one
+//                  kernel.
+constexpr uint32_t kLeafItem = 0x80000000u;  // frontier entry is a leaf index
+constexpr uint32_t kRefitThreads = 1024;
+constexpr uint32_t kLeavesPerThread = kRefitLeaves / kRefitThreads;
+
+__global__ void __launch_bounds__(kRefitThreads) k_refit_local(const float4* __restrict__ sph, uint32_t n,
+                                                               const uint32_t* __restrict__ child,
+                                                               const uint32_t* __restrict__ parent_int,
+                                                               const uint32_t* __restrict__ parent_leaf,
+                                                               float* __restrict__ boxes,
+                                                               uint32_t* __restrict__ height,
+                                                               uint4* __restrict__ nodes,
+                                                               uint32_t* __restrict__ frontier,
+                                                               uint32_t* __restrict__ fcount)
+{
+    __shared__ uint32_t s_visit[kRefitLeaves];
+    __shared__ uint32_t s_child[kRefitLeaves];
+    __shared__ uint32_t s_up[kRefitLeaves];
+    __shared__ uint32_t s_h[kRefitLeaves];
+    __shared__ float s_box[6][kRefitLeaves];
+    const uint32_t tid = threadIdx.x, base = blockIdx.x * kRefitLeaves;
+    for (uint32_t i = tid; i < kRefitLeaves; i += kRefitThreads) {
+        s_visit[i] = 0u;
+        const uint32_t g = base + i;
+        if (g + 1u < n) {  // internal nodes are 0 .. n-2
+            s_child[i] = child[g];
+            s_up[i] = parent_int[g];
         }
+    }
+    __syncthreads();
+    for (uint32_t r = 0; r < kLeavesPerThread; r++) {
+        const uint32_t leaf = base + r * kRefitThreads + tid;
+        if (leaf >= n) break;
+        uint32_t up = parent_leaf[leaf];
+        uint32_t done = leaf | kLeafItem;  // the highest node this climb has finished
         float nb[6];
-        for (int k = 0; k < 3; k++) {
-            nb[k] = fminf(cb[0][k], cb[1][k]);
-            nb[3 + k] = fmaxf(cb[0][3 + k], cb[1][3 + k]);
+        uint32_t nh = 0;
+        while (up & kLocalBit) {
+            const uint32_t p = up & ~kLocalBit, slot = p - base;
+            if (__hip_atomic_fetch_add(&s_visit[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+                break;
+            const uint32_t cw = s_child[slot];
+            const uint32_t g = cw & 0x3FFFFFFFu, flags = cw >> 30;
+            float cb[2][6];
+            uint32_t h = 0;
+            for (int c = 0; c < 2; c++) {
+                const uint32_t ch = g + (uint32_t)c;
+                if ((flags >> c) & 1u) {
+                    leaf_box(sph[ch], cb[c]);
+                } else {
+                    const uint32_t cs = ch - base;
+                    for (int k = 0; k < 6; k++)
+                        cb[c][k] = __hip_atomic_load(&s_box[k][cs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    h = max(h, __hip_atomic_load(&s_h[cs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+                }
+            }
+            pack_node(p, cb, g, flags, nb, nodes);
+            nh = h + 1u;
+            for (int k = 0; k < 6; k++)
+                __hip_atomic_store(&s_box[k][slot], nb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&s_h[slot], nh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            done = p;
+            if (p == 0u) break;  // the whole tree was local
+            up = s_up[slot];
         }
-        for (int k = 0; k < 6; k++)
-            __hip_atomic_store(&boxes[p * 6 + k], nb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&height[p], h + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t meta = 0, q[3];
-        float o[3];
-        for (int k = 0; k < 3; k++) {
-            float lo = nb[k], hi = nb[3 + k];
-            uint32_t e = pick_exp(lo, hi);
-            float s = __uint_as_float(e << 23);
-            meta |= e << (8 * k);
-            uint32_t l0 = quant_lo(cb[0][k], lo, s), h0 = quant_hi(cb[0][3 + k], lo, s);
-            uint32_t l1 = quant_lo(cb[1][k], lo, s), h1 = quant_hi(cb[1][3 + k], lo, s);
-            o[k] = lo;
-            q[k] = l0 | (h0 << 8) | (l1 << 16) | (h1 << 24);
+        if (up & kLocalBit) {
+            // stopped inside the workgroup: first arrival, or the local root
+            if (done != 0u) continue;
         }
-        meta |= flags << 24;
-        nodes[2 * p] = make_uint4(__float_as_uint(o[0]), __float_as_uint(o[1]), __float_as_uint(o[2]), meta);
-        nodes[2 * p + 1] = make_uint4(q[0], q[1], q[2], g);
-        if (p == 0u) return;
-        p = parent_int[p];
+        if (!(done & kLeafItem)) {  // a finished local node: publish its box and height
+            for (int k = 0; k < 6; k++) boxes[done * 6 + k] = nb[k];
+            height[done] = nh;
+        }
+        if (done != 0u) frontier[atomicAdd(fcount, 1u)] = done;
+    }
+}
+
+constexpr uint32_t kTopBlocks = 64;
+
+__global__ void __launch_bounds__(kRefitThreads) k_refit_top(const float4* __restrict__ sph,
+                                                    const uint32_t* __restrict__ child,
+                                                    const uint32_t* __restrict__ parent_int,
+                                                    const uint32_t* __restrict__ parent_leaf,
+                                                    const uint32_t* __restrict__ frontier,
+                                                    const uint32_t* __restrict__ fcount, uint32_t* visit,
+                                                    float* boxes, uint32_t* height, uint4* __restrict__ nodes)
+{
+    const uint32_t count = *fcount;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t item = frontier[i];
+        uint32_t p = ((item & kLeafItem) ? parent_leaf[item & ~kLeafItem] : parent_int[item]) & ~kLocalBit;
+        for (;;) {
+            if (__hip_atomic_fetch_add(&visit[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0u) break;
+            const uint32_t cw = child[p];
+            const uint32_t g = cw & 0x3FFFFFFFu, flags = cw >> 30;
+            float cb[2][6];
+            uint32_t h = 0;
+            for (int c = 0; c < 2; c++) {
+                const uint32_t ch = g + (uint32_t)c;
+                if ((flags >> c) & 1u) {
+                    leaf_box(sph[ch], cb[c]);
+                } else {
+                    for (int k = 0; k < 6; k++)
+                        cb[c][k] = __hip_atomic_load(&boxes[ch * 6 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    h = max(h, __hip_atomic_load(&height[ch], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                }
+            }
+            float nb[6];
+            pack_node(p, cb, g, flags, nb, nodes);
+            for (int k = 0; k < 6; k++) __hip_atomic_store(&boxes[p * 6 + k], nb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&height[p], h + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (p == 0u) break;
+            p = parent_int[p] & ~kLocalBit;
+        }
     }
 }
 
@@ -430,7 +565,7 @@ vzStatus lbvh_alloc(vzScene s)
         VZ_TRY(al((void**)&s->d_keys[i], sizeof(uint32_t) * n));
         VZ_TRY(al((void**)&s->d_vals[i], sizeof(uint32_t) * n));
     }
-    VZ_TRY(al((void**)&s->d_hist, sizeof(uint32_t) * 256 * s->hist_blocks));
+    VZ_TRY(al((void**)&s->d_hist, sizeof(uint32_t) * 256 * (s->hist_blocks + 1)));
     VZ_TRY(al((void**)&s->d_bounds, sizeof(float) * (6 * 1024 + 16)));
     VZ_TRY(al((void**)&s->d_sph, sizeof(float4) * n));
     VZ_TRY(al((void**)&s->d_leaf_info, sizeof(uint32_t) * n));
@@ -442,6 +577,8 @@ vzStatus lbvh_alloc(vzScene s)
     VZ_TRY(al((void**)&s->d_parent_leaf, sizeof(uint32_t) * n));
     VZ_TRY(al((void**)&s->d_visit, sizeof(uint32_t) * ni));
     VZ_TRY(al((void**)&s->d_height, sizeof(uint32_t) * ni));
+    VZ_TRY(al((void**)&s->d_frontier, sizeof(uint32_t) * n));
+    VZ_TRY(al((void**)&s->d_fcount, 256));
     s->device_bytes = bytes;
     return VZ_SUCCESS;
 }
@@ -451,7 +588,7 @@ void lbvh_free(vzScene s)
     void* ptrs[] = {s->d_sph_in, s->d_mat_in, s->d_keys[0], s->d_keys[1], s->d_vals[0], s->d_vals[1],
                     s->d_hist, s->d_bounds, s->d_sph, s->d_leaf_info, s->d_mat, s->d_nodes,
                     s->d_boxes, s->d_child, s->d_parent_int, s->d_parent_leaf, s->d_visit,
-                    s->d_height};
+                    s->d_height, s->d_frontier, s->d_fcount};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -461,9 +598,13 @@ static vzStatus refit_core(vzScene s, hipStream_t st)
     uint32_t n = s->n;
     if (n < 2) return VZ_SUCCESS;
     hipLaunchKernelGGL(k_zero, dim3(blocks_for(n - 1, kBlk)), dim3(kBlk), 0, st, s->d_visit, n - 1);
-    hipLaunchKernelGGL(k_refit, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph, n, s->d_child,
-                       s->d_parent_int, s->d_parent_leaf, s->d_visit, s->d_boxes, s->d_height,
-                       s->d_nodes);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlk), 0, st, s->d_fcount, 1u);
+    hipLaunchKernelGGL(k_refit_local, dim3(blocks_for(n, kRefitLeaves)), dim3(kRefitThreads), 0, st, s->d_sph,
+                       n, s->d_child, s->d_parent_int, s->d_parent_leaf, s->d_boxes, s->d_height, s->d_nodes,
+                       s->d_frontier, s->d_fcount);
+    hipLaunchKernelGGL(k_refit_top, dim3(kTopBlocks), dim3(kRefitThreads), 0, st, s->d_sph, s->d_child,
+                       s->d_parent_int, s->d_parent_leaf, s->d_frontier, s->d_fcount, s->d_visit, s->d_boxes,
+                       s->d_height, s->d_nodes);
     VZ_HIP_TRY(hipGetLastError());
     uint32_t h = 0;
     VZ_HIP_TRY(hipMemcpyAsync(&h, s->d_height, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -487,17 +628,18 @@ vzStatus lbvh_build(vzScene s, hipStream_t st)
     float* partial = s->d_bounds;
     float* fin = s->d_bounds + 6 * 1024;
     hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(kBlk), 0, st, s->d_sph_in, n, partial);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, st, partial, nb, fin);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlk), 0, st, partial, nb, fin);
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in, n, fin,
                        s->d_keys[0], s->d_vals[0]);
     uint32_t tiles = s->hist_blocks;
+    uint32_t* dtot = s->d_hist + 256 * tiles;
     for (int pass = 0; pass < 4; pass++) {
         int src = pass & 1, dst = src ^ 1;
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src], n, pass * 8,
                            tiles, s->d_hist);
-        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, s->d_hist, 256 * tiles);
+        hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(kBlk), 0, st, s->d_hist, tiles, dtot);
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src],
-                           s->d_vals[src], n, pass * 8, tiles, s->d_hist, s->d_keys[dst], s->d_vals[dst]);
+                           s->d_vals[src], n, pass * 8, tiles, s->d_hist, dtot, s->d_keys[dst], s->d_vals[dst]);
     }
     s->d_morton = s->d_keys[0];  // four passes: result back in buffer 0
     s->d_leaf_prim = s->d_vals[0];
