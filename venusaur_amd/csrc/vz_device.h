@@ -25,6 +25,18 @@ struct v3 {
     float x, y, z;
 };
 
+// Loads through a global-address-space pointer.  Pointers read out of a
+// struct in memory are generic, so the compiler would emit a flat load, whose
+// wait also covers LDS traffic; every buffer this library hands its kernels is
+// in device (global) memory.
+typedef float raw_float4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const raw_float4 global_float4;
+VZ_DEV float4 ldg(const float4* p)
+{
+    const raw_float4 v = *__builtin_bit_cast(global_float4*, p);  // a C-style cast would be dropped
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 VZ_DEV v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
 VZ_DEV v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
 VZ_DEV v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }

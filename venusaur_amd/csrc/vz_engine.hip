@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         my_free += fr[q];
     }
     uint32_t freeq;
-    const uint32_t free_pre = block_excl_scan(my_free, s_scan, &freeq);
+    (void)block_excl_scan(my_free, s_scan, &freeq);
     // 2. per request: passes whose paths all ended, and the camera-ray quota
     uint32_t acc_first = 0, acc_cnt = 0, quota = 0, npix = 0;
     bool running = false, still = false;
@@ -374,15 +374,51 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         plan->regen_prefix[posR] = preR;
         plan->regen_base[posR] = next_item;
     }
-    // 3. split the granted camera rays over the slices, in slice order
-    uint32_t run = free_pre;
+    // 3. split the granted camera rays over the slices so that every slice
+    // ends up near the same ray count: the next step launch lasts as long as
+    // its fullest slice.  Level L = ceil(rays after regen / NB) <= R; a slice
+    // below L gets a share of the new rays proportional to its deficit.
+    const unsigned long long held = (unsigned long long)NB * R - freeq;
+    const uint32_t level = (uint32_t)((held + totR + NB - 1u) / NB);
+    uint32_t def[kSlicesPerThread], my_def = 0;
     for (uint32_t q = 0; q < kSlicesPerThread; q++) {
-        uint32_t sb = r + q * 1024u;
+        const uint32_t sb = r + q * 1024u;
+        const uint32_t c = R - fr[q];
+        def[q] = (sb < NB && level > c) ? level - c : 0u;  // <= fr[q] since level <= R
+        my_def += def[q];
+    }
+    uint32_t tdef;
+    (void)block_excl_scan(my_def, s_scan, &tdef);  // tdef >= totR
+    uint32_t g[kSlicesPerThread], my_g = 0, my_short = 0;
+    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
+        g[q] = tdef ? (uint32_t)((unsigned long long)totR * def[q] / tdef) : 0u;
+        my_g += g[q];
+        my_short += g[q] < def[q] ? 1u : 0u;
+    }
+    uint32_t tg, tshort;
+    (void)block_excl_scan(my_g, s_scan, &tg);
+    const uint32_t short_pre = block_excl_scan(my_short, s_scan, &tshort);
+    // the rounding leftover (< number of short slices): one more ray each to
+    // the first short slices
+    const uint32_t left = totR - tg;
+    uint32_t k = short_pre;
+    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
+        if (g[q] < def[q]) {
+            if (k < left) g[q]++;
+            k++;
+        }
+    }
+    uint32_t my_final = 0;
+    for (uint32_t q = 0; q < kSlicesPerThread; q++) my_final += g[q];
+    uint32_t tfinal;
+    uint32_t run = block_excl_scan(my_final, s_scan, &tfinal);
+    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
+        const uint32_t sb = r + q * 1024u;
         if (sb < NB) {
             plan->blk_first[sb] = run;
-            plan->blk_count[sb] = totR > run ? min(fr[q], totR - run) : 0u;
+            plan->blk_count[sb] = g[q];
         }
-        run += fr[q];
+        run += g[q];
     }
     if (r == 0) {
         plan->n_acc = nA;

@@ -99,12 +99,33 @@ struct Traversal {
         return false;
     }
 
+    // Sphere test of leaf `leaf`; keeps the lexicographic minimum of (t, prim).
+    VZ_DEV void test_leaf(const SceneView& sc, uint32_t leaf, uint32_t& ns)
+    {
+        float t;
+        if (COUNT) ns++;
+        if (hit_sphere(o, d, tmin, best, ldg(sc.spheres + leaf), t)) {
+            if (t < best) {
+                best = t;
+                bleaf = leaf;
+                bprim = 0xFFFFFFFFu;  // looked up lazily on a tie
+            } else {
+                const uint32_t pid = sc.leaf_info[leaf] & 0x3FFFFFFFu;
+                if (bprim == 0xFFFFFFFFu && bleaf != 0xFFFFFFFFu) bprim = sc.leaf_info[bleaf] & 0x3FFFFFFFu;
+                if (pid < bprim) {
+                    best = t;
+                    bleaf = leaf;
+                    bprim = pid;
+                }
+            }
+        }
+    }
+
     // Visits one node; returns true when the traversal has finished.
     template <class NodeSrc>
     VZ_DEV bool step(const SceneView& sc, const NodeSrc& nodes, uint32_t* lds_node, float* lds_t,
                      uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns)
     {
-        if (budget-- == 0u) return true;
         uint4 A, B;
         nodes.fetch(node, A, B);
         if (COUNT) nv++;
@@ -113,7 +134,6 @@ struct Traversal {
         const uint32_t meta = A.w;
         const uint32_t qw[3] = {B.x, B.y, B.z};
         const uint32_t g = B.w;
-        if (g + 1u >= sc.n) return true;
         float tn[2];
         bool hit[2];
 #pragma unroll
@@ -132,31 +152,15 @@ struct Traversal {
             tn[c] = lo_t;
             hit[c] = lo_t <= hi_t * kRobust;
         }
+        // guards, checked after the box arithmetic so both node loads issue
+        // together: children must be valid indices, and a well-formed tree
+        // visits each of its n-1 internal nodes at most once
+        if (g + 1u >= sc.n || budget-- == 0u) return true;
         const uint32_t flags = meta >> 24;
+        // leaf children, in child order
 #pragma unroll
-        for (int c = 0; c < 2; c++) {
-            if (hit[c] && ((flags >> c) & 1u)) {
-                uint32_t leaf = g + (uint32_t)c;
-                float t;
-                if (COUNT) ns++;
-                if (hit_sphere(o, d, tmin, best, sc.spheres[leaf], t)) {
-                    if (t < best) {
-                        best = t;
-                        bleaf = leaf;
-                        bprim = 0xFFFFFFFFu;  // looked up lazily on a tie
-                    } else {
-                        uint32_t pid = sc.leaf_info[leaf] & 0x3FFFFFFFu;
-                        if (bprim == 0xFFFFFFFFu && bleaf != 0xFFFFFFFFu)
-                            bprim = sc.leaf_info[bleaf] & 0x3FFFFFFFu;
-                        if (pid < bprim) {
-                            best = t;
-                            bleaf = leaf;
-                            bprim = pid;
-                        }
-                    }
-                }
-            }
-        }
+        for (int c = 0; c < 2; c++)
+            if (hit[c] && ((flags >> c) & 1u)) test_leaf(sc, g + (uint32_t)c, ns);
         const bool in0 = hit[0] && !(flags & 1u);
         const bool in1 = hit[1] && !(flags & 2u);
         if (in1 && g + 2u >= sc.n) return true;  // internal child g+1 must be <= n-2
