@@ -77,14 +77,18 @@ struct Plan {
     uint32_t acc_cnt[kMaxRequests];
     uint32_t acc_prefix[kMaxRequests];
     uint32_t regen_req[kMaxRequests];
-    uint32_t regen_prefix[kMaxRequests];
-    unsigned long long regen_base[kMaxRequests];
-    uint32_t blk_first[kMaxBlocks];   // first granted item of each slice
+    uint32_t regen_prefix[kMaxRequests];   // first granted index of each regen entry
+    uint32_t regen_cnt[kMaxRequests];      // items granted to it (may span several passes)
+    uint32_t run_prefix[kMaxRequests];     // first pixel run of it: min(cnt, npad) runs each
+    unsigned long long regen_base[kMaxRequests];  // its first item (pass * npad + padded pixel)
+    uint32_t total_runs;
+    uint32_t blk_first[kMaxBlocks];   // first granted index of each slice
     uint32_t blk_count[kMaxBlocks];   // granted items for each slice
+    uint32_t blk_pos0[kMaxBlocks];    // the slice's ray count before regen
 };
 
 struct EngineConfig {
-    uint32_t queue_capacity = 1u << 22;
+    uint32_t queue_capacity = 1u << 23;  // rays in flight; 8M suits both LDS-resident and 1M-sphere scenes
     uint32_t tile_w = 32, tile_h = 2;   // one wave of camera rays = 32x2 pixels
     uint32_t iters_per_graph = 16;
     uint32_t refill_lanes = 0;          // 0: per-variant default (kernel side)

@@ -298,13 +298,16 @@ vzStatus vzServerDestroy(vzServer srv)
 vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStream stream, vzTuning* out)
 {
     if (!ctx || !items || n == 0 || !out) return set_error(VZ_ERROR_INVALID_VALUE, "NULL argument or empty batch");
-    // reduced-sample copy of the batch: same scenes, sizes and depths
+    // reduced-sample copy of the batch: same scenes, sizes and depths, a
+    // quarter of the samples (at least min(spp, 4)) so that each request's
+    // ring of passes in flight stays close to the full batch's
     std::vector<vzBatchItem> probe(items, items + n);
     for (auto& it : probe) {
-        it.params.samples_per_pixel = std::min<uint32_t>(it.params.samples_per_pixel, 4u);
+        const uint32_t spp = it.params.samples_per_pixel;
+        it.params.samples_per_pixel = std::max(std::min(spp, 4u), spp / 4u);
         it.params.flags &= ~(uint32_t)(VZ_COUNT_VISITS | VZ_TIME_STAGES);
     }
-    const uint32_t caps[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
     const uint32_t tiles[][2] = {{8, 8}, {16, 4}, {32, 2}, {64, 1}};
     vzTuning best{};
     double best_rate = -1.0;
