@@ -356,6 +356,41 @@ static uint32_t morton_of(const vzo_sphere* s, const float lo[3], const float sc
     return (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
 }
 
+#define VZO_LARGE_KEY 0x40000000u
+
+static inline int is_large(const vzo_sphere* s, float thr) { return 2.0f * fabsf(s->r) > thr; }
+
+/* Centroid bounds lo/hi and Morton scale sc over the spheres with
+ * 2|r| <= thr (all of them when thr < 0).  An axis with no sphere counted
+ * gets lo = hi = 0.  iso = 0: sc = ext > 0 ? 1024/ext : 0 per axis; iso = 1:
+ * one scale 1024/max ext for every axis (cubic cells, so a thin slab of
+ * spheres is not split along its thin axis as often as along the others). */
+static void centroid_bounds(const vzo_sphere* s, uint32_t n, float thr, int iso, float lo[3], float hi[3],
+                            float sc[3])
+{
+    for (int k = 0; k < 3; k++) {
+        lo[k] = 3.4e38f;
+        hi[k] = -3.4e38f;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        if (thr >= 0.0f && is_large(&s[i], thr)) continue;
+        const float c[3] = {s[i].cx, s[i].cy, s[i].cz};
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], c[k]);
+            hi[k] = fmaxf(hi[k], c[k]);
+        }
+    }
+    float extmax = 0.0f;
+    for (int k = 0; k < 3; k++) {
+        if (lo[k] > hi[k]) lo[k] = hi[k] = 0.0f;
+        const float ext = hi[k] - lo[k];
+        sc[k] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+        extmax = fmaxf(extmax, ext);
+    }
+    if (iso)
+        for (int k = 0; k < 3; k++) sc[k] = extmax > 0.0f ? 1024.0f / extmax : 0.0f;
+}
+
 static inline int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
 
 /* Karras 2012 delta: common prefix length of keys i and j, with the index
@@ -489,22 +524,23 @@ vzo_bvh* vzo_bvh_build(const vzo_sphere* s, uint32_t n)
     uint32_t ni = n > 1 ? n - 1 : 0;
     b->nodes = (vzo_node*)calloc(ni ? ni : 1, sizeof(vzo_node));
     b->boxes = (float*)calloc((ni ? ni : 1) * 6, sizeof(float));
-    /* centroid bounds */
-    float lo[3] = {s[0].cx, s[0].cy, s[0].cz}, hi[3] = {s[0].cx, s[0].cy, s[0].cz};
-    for (uint32_t i = 1; i < n; i++) {
-        float c[3] = {s[i].cx, s[i].cy, s[i].cz};
-        for (int k = 0; k < 3; k++) {
-            lo[k] = fminf(lo[k], c[k]);
-            hi[k] = fmaxf(hi[k], c[k]);
-        }
-    }
-    float sc[3];
-    for (int k = 0; k < 3; k++) {
-        float ext = hi[k] - lo[k];
-        sc[k] = ext > 0.0f ? 1024.0f / ext : 0.0f;
-    }
+    /* Centroid bounds of all spheres (grid A), the size threshold, and the
+     * centroid bounds of the spheres at or below it (grid B).  A sphere
+     * whose diameter exceeds a quarter of the largest centroid extent is
+     * "large": its key is its Morton code on grid A with bit 30 set, so the
+     * first split of the tree separates it from the rest, and its huge box
+     * (the r = 1000 ground) does not inflate a whole root-to-leaf path.  The
+     * other spheres get their Morton code on grid B: their own bounds, cubic
+     * cells. */
+    float loA[3], hiA[3], scA[3], loB[3], hiB[3], scB[3];
+    centroid_bounds(s, n, -1.0f, 0, loA, hiA, scA);
+    float extmax = 0.0f;
+    for (int k = 0; k < 3; k++) extmax = fmaxf(extmax, hiA[k] - loA[k]);
+    const float thr = 0.25f * extmax;
+    centroid_bounds(s, n, thr, 1, loB, hiB, scB);
     for (uint32_t i = 0; i < n; i++) {
-        b->morton[i] = morton_of(&s[i], lo, sc);
+        b->morton[i] = is_large(&s[i], thr) ? (morton_of(&s[i], loA, scA) | VZO_LARGE_KEY)
+                                            : morton_of(&s[i], loB, scB);
         b->leaf_prim[i] = i;
     }
     radix_sort(b->morton, b->leaf_prim, n);
@@ -731,9 +767,11 @@ void vzo_trace_brute(const vzo_sphere* s, uint32_t nsph, const vzo_ray* rays, ui
         float best = r->tmax;
         uint32_t bp = VZO_MISS;
         v3_f32 o = mk_f32(r->ox, r->oy, r->oz), d = mk_f32(r->dx, r->dy, r->dz);
+        float inv[3];
+        inv_dir_f32(d, inv);
         for (uint32_t j = 0; j < nsph; j++) {
             float t;
-            if (hit_sphere_f32(o, d, r->tmin, best, s[j].cx, s[j].cy, s[j].cz, s[j].r, &t)) {
+            if (hit_gated_f32(o, d, inv, r->tmin, best, &s[j], &t)) {
                 if (t < best || j < bp) {
                     best = t;
                     bp = j;
@@ -753,9 +791,11 @@ void vzo_trace_brute_f64(const vzo_sphere* s, uint32_t nsph, const vzo_ray* rays
         double best = r->tmax;
         uint32_t bp = VZO_MISS;
         v3_f64 o = mk_f64(r->ox, r->oy, r->oz), d = mk_f64(r->dx, r->dy, r->dz);
+        double inv[3];
+        inv_dir_f64(d, inv);
         for (uint32_t j = 0; j < nsph; j++) {
             double t;
-            if (hit_sphere_f64(o, d, r->tmin, best, s[j].cx, s[j].cy, s[j].cz, s[j].r, &t)) {
+            if (hit_gated_f64(o, d, inv, r->tmin, best, &s[j], &t)) {
                 if (t < best || j < bp) {
                     best = t;
                     bp = j;

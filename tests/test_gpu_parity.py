@@ -100,6 +100,17 @@ def test_trace_rayset_1m(ctx, book):
     assert st.sphere_tests == rs["sphere_tests"]
 
 
+def test_trace_grazing_rays_gated(ctx):
+    say("case: trace grazing rays on the 20000-sphere box scene")
+    s, m = api.box_scene(20000, 2024)
+    sc = api.Scene(ctx, s, m)
+    rays = np.load(os.path.join(GOLDEN, "box_grazing_rays.npy"))
+    fix = json.load(open(os.path.join(GOLDEN, "box_grazing.json")))
+    got, _ = _trace(sc, rays)
+    assert [float(x) for x in got["t"]] == fix["t"] and [int(x) for x in got["prim"]] == fix["prim"]
+    assert got.tobytes() == O.BVH(s).trace(rays)[0].tobytes()
+
+
 def test_trace_box_scene_vs_brute(ctx):
     say("case: trace box scene vs brute force")
     s, m = api.box_scene(50000, 77)

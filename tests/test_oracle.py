@@ -411,3 +411,44 @@ def test_plant_a_mistake_is_caught():
     hit_prims = np.unique(good["prim"][good["prim"] != O.MISS])
     s2["r"][hit_prims] *= np.float32(0.9)
     assert O.trace_brute(s2, rays).tobytes() != good.tobytes()
+
+
+# ---- box-gated sphere test ----------------------------------------------------------
+def test_grazing_roots_are_gated_by_the_sphere_box():
+    """Five rays on which the ungated float32 sphere test reports a root on a
+    tiny distant sphere (r ~ 0.01-0.03 at 70-250 units) that the ray passes
+    outside: |oc|^2 - r^2 loses r^2 to rounding.  The root lies outside the
+    sphere's box, so the gated test drops it, and LBVH traversal equals brute
+    force (which it did not while the test was ungated)."""
+    s, _ = O.box_scene(20000, 2024)
+    rays = np.load(os.path.join(GOLDEN, "box_grazing_rays.npy"))
+    fix = json.load(open(os.path.join(GOLDEN, "box_grazing.json")))
+    for ray, p in zip(rays, fix["spurious_prim"]):
+        o = [ray["ox"], ray["oy"], ray["oz"]]
+        d = [ray["dx"], ray["dy"], ray["dz"]]
+        t = _hit32(o, d, s[p])
+        assert t is not None  # the ungated test reports a root ...
+        pt = np.asarray(o, np.float64) + t * np.asarray(d, np.float64)
+        c = np.array([s[p]["cx"], s[p]["cy"], s[p]["cz"]], np.float64)
+        r = abs(float(s[p]["r"]))
+        assert np.any(np.abs(pt - c) > r * 1.01)  # ... at a point outside the sphere's box
+    hb, _, _ = O.BVH(s).trace(rays)
+    hf = O.trace_brute(s, rays)
+    assert hb.tobytes() == hf.tobytes()
+    assert [float(x) for x in hb["t"]] == fix["t"] and [int(x) for x in hb["prim"]] == fix["prim"]
+    assert not set(int(x) for x in hb["prim"]) & set(fix["spurious_prim"])
+
+
+def test_box_scene_bvh_equals_brute_force():
+    """The closest hit does not depend on the tree: LBVH traversal equals
+    brute force bit for bit on the integrator's rays of a box scene of tiny
+    spheres, where ungated grazing roots used to make them differ."""
+    s, m = O.box_scene(20000, 2024)
+    W, H = 160, 90
+    cam = O.camera_lookat([0, 15, -120], [0, 0, 0], 40.0, float(np.float32(W) / np.float32(H)), 0.0, 10.0)
+    p = O.make_params(W, H, 16, 16, cam)
+    b = O.BVH(s)
+    rays = O.make_rayset(s, m, p, 24000, 6000, 5, bvh=b)
+    hb, _, _ = b.trace(rays)
+    hf = O.trace_brute(s, rays)
+    assert hb.tobytes() == hf.tobytes()

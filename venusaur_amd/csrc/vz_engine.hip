@@ -74,7 +74,7 @@ __device__ __forceinline__ void terminate(const ReqDesc& D, uint32_t slot, uint3
 // the launch belongs to scene 0 (the scene descriptor lives in scalars);
 // LDSN: that scene's node array is staged in LDS.
 template <bool COUNT, bool LDSN, bool SINGLE>
-__global__ void __launch_bounds__(kStepBlk) k_wf_step(const SceneView* __restrict__ scenes,
+__global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6))) k_wf_step(const SceneView* __restrict__ scenes,
                                                       const ReqDesc* __restrict__ req,
                                                       const float4* __restrict__ q_o,
                                                       const float4* __restrict__ q_d,

@@ -29,6 +29,7 @@ constexpr int kBlk = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBlk * kItems;  // keys per radix tile
 constexpr uint32_t kLocalBit = 0x80000000u;  // parent link flag (see k_karras)
+constexpr uint32_t kLargeKey = 0x40000000u;  // sort-key bit of a large sphere (see k_morton)
 constexpr uint32_t kRefitLeaves = 2048;       // leaves per k_refit_local workgroup
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -38,13 +39,18 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
 }
 
 // ---- 1. bounds ---------------------------------------------------------------
+// Centroid bounds, per workgroup.  With `grid_a` set only the spheres at or
+// below its size threshold (2|r| <= grid_a[9]) count.
 __global__ void __launch_bounds__(kBlk) k_bounds_partial(const float4* __restrict__ sph, uint32_t n,
+                                                         const float* __restrict__ grid_a,
                                                          float* __restrict__ partial)
 {
     __shared__ float s[6][kBlk];
     float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    const float thr = grid_a ? grid_a[9] : 0.0f;
     for (uint32_t i = blockIdx.x * kBlk + threadIdx.x; i < n; i += gridDim.x * kBlk) {
         float4 c = sph[i];
+        if (grid_a && 2.0f * fabsf(c.w) > thr) continue;
         lo[0] = fminf(lo[0], c.x); lo[1] = fminf(lo[1], c.y); lo[2] = fminf(lo[2], c.z);
         hi[0] = fmaxf(hi[0], c.x); hi[1] = fmaxf(hi[1], c.y); hi[2] = fmaxf(hi[2], c.z);
     }
@@ -62,10 +68,12 @@ __global__ void __launch_bounds__(kBlk) k_bounds_partial(const float4* __restric
     if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// out: lo[3], hi[3], scale[3] with scale = ext > 0 ? 1024/ext : 0.  min and
-// max are exact, so the reduction order does not change the result.
+// out: lo[3], hi[3], scale[3] with scale = ext > 0 ? 1024/ext : 0 (iso: one
+// scale 1024/max ext for every axis), and the large-sphere threshold
+// out[9] = max extent / 4; an axis that no sphere reached gets lo = hi = 0.  min and max are exact, so the reduction order
+// does not change the result.
 __global__ void __launch_bounds__(kBlk) k_bounds_final(const float* __restrict__ partial, uint32_t nb,
-                                                       float* __restrict__ out)
+                                                       uint32_t iso, float* __restrict__ out)
 {
     __shared__ float s[6][kBlk];
     float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
@@ -84,12 +92,20 @@ __global__ void __launch_bounds__(kBlk) k_bounds_final(const float* __restrict__
             }
         __syncthreads();
     }
-    if (threadIdx.x < 3) {
-        const int k = (int)threadIdx.x;
-        const float l = s[k][0], h = s[3 + k][0], ext = h - l;
-        out[k] = l;
-        out[3 + k] = h;
-        out[6 + k] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+    if (threadIdx.x == 0) {
+        float extmax = 0.0f;
+        for (int k = 0; k < 3; k++) {
+            float l = s[k][0], h = s[3 + k][0];
+            if (l > h) l = h = 0.0f;
+            const float ext = h - l;
+            out[k] = l;
+            out[3 + k] = h;
+            out[6 + k] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+            extmax = fmaxf(extmax, ext);
+        }
+        if (iso)  // one scale for every axis: cubic Morton cells
+            for (int k = 0; k < 3; k++) out[6 + k] = extmax > 0.0f ? 1024.0f / extmax : 0.0f;
+        out[9] = 0.25f * extmax;
     }
 }
 
@@ -103,13 +119,24 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v)
     return v;
 }
 
+// Sort key of sphere i.  A sphere whose diameter exceeds grid_a's threshold
+// (a quarter of the largest centroid extent, e.g. the r = 1000 ground) is
+// "large": its key is its Morton code on grid A (all centroids) with bit 30
+// set, so the first split of the tree separates it from the rest and its box
+// does not inflate a whole root-to-leaf path.  Every other sphere gets its
+// Morton code on grid B: the centroids of the small spheres only, with cubic
+// cells, so a thin slab of spheres is not split along its thin axis as often
+// as along the others.
 __global__ void __launch_bounds__(kBlk) k_morton(const float4* __restrict__ sph, uint32_t n,
-                                                 const float* __restrict__ bnd,
+                                                 const float* __restrict__ grid_a,
+                                                 const float* __restrict__ grid_b,
                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     uint32_t i = blockIdx.x * kBlk + threadIdx.x;
     if (i >= n) return;
     float4 c = sph[i];
+    const bool large = 2.0f * fabsf(c.w) > grid_a[9];
+    const float* bnd = large ? grid_a : grid_b;
     float cc[3] = {c.x, c.y, c.z};
     uint32_t q[3];
     for (int k = 0; k < 3; k++) {
@@ -118,7 +145,7 @@ __global__ void __launch_bounds__(kBlk) k_morton(const float4* __restrict__ sph,
         qi = qi < 0 ? 0 : (qi > 1023 ? 1023 : qi);
         q[k] = (uint32_t)qi;
     }
-    keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]) | (large ? kLargeKey : 0u);
     vals[i] = i;
 }
 
@@ -566,7 +593,7 @@ vzStatus lbvh_alloc(vzScene s)
         VZ_TRY(al((void**)&s->d_vals[i], sizeof(uint32_t) * n));
     }
     VZ_TRY(al((void**)&s->d_hist, sizeof(uint32_t) * 256 * (s->hist_blocks + 1)));
-    VZ_TRY(al((void**)&s->d_bounds, sizeof(float) * (6 * 1024 + 16)));
+    VZ_TRY(al((void**)&s->d_bounds, sizeof(float) * (6 * 1024 + 32)));  // partials, grid A, grid B
     VZ_TRY(al((void**)&s->d_sph, sizeof(float4) * n));
     VZ_TRY(al((void**)&s->d_leaf_info, sizeof(uint32_t) * n));
     VZ_TRY(al((void**)&s->d_mat, sizeof(float4) * n));
@@ -627,9 +654,12 @@ vzStatus lbvh_build(vzScene s, hipStream_t st)
     VZ_HIP_TRY(hipEventRecord(e0, st));
     float* partial = s->d_bounds;
     float* fin = s->d_bounds + 6 * 1024;
-    hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(kBlk), 0, st, s->d_sph_in, n, partial);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlk), 0, st, partial, nb, fin);
-    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in, n, fin,
+    float* grid_b = fin + 16;
+    hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(kBlk), 0, st, s->d_sph_in, n, nullptr, partial);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlk), 0, st, partial, nb, 0u, fin);
+    hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(kBlk), 0, st, s->d_sph_in, n, fin, partial);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlk), 0, st, partial, nb, 1u, grid_b);
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in, n, fin, grid_b,
                        s->d_keys[0], s->d_vals[0]);
     uint32_t tiles = s->hist_blocks;
     uint32_t* dtot = s->d_hist + 256 * tiles;

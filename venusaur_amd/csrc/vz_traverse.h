@@ -73,21 +73,21 @@ struct Traversal {
         best = tmax_;
         bleaf = 0xFFFFFFFFu;
         bprim = 0xFFFFFFFFu;
-        if (sc.n == 1u) {
-            float t;
-            if (COUNT) ns++;
-            if (hit_sphere(o, d, tmin, best, sc.spheres[0], t)) {
-                best = t;
-                bleaf = 0;
-            }
-            return true;
-        }
         const float dd[3] = {d.x, d.y, d.z};
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             float dk = dd[k];
             if (!(fabsf(dk) > 1e-20f)) dk = copysignf(1e-20f, dk);
             inv[k] = 1.0f / dk;
+        }
+        if (sc.n == 1u) {
+            float t;
+            if (COUNT) ns++;
+            if (hit_gated(ldg(sc.spheres), t)) {
+                best = t;
+                bleaf = 0;
+            }
+            return true;
         }
         sp = 0;
         bottom = 0;
@@ -99,12 +99,36 @@ struct Traversal {
         return false;
     }
 
+    // The sphere test (root in [tmin, best]) gated by the sphere's own box:
+    // the root counts only if the ray enters [c - |r|, c + |r|] by t, by the
+    // same robust slab test as the nodes.  OptiX runs an intersection program
+    // only for primitives whose box the ray enters; the gate also drops the
+    // spurious grazing roots that float rounding of |oc|^2 - r^2 gives tiny
+    // distant spheres, and it makes the closest hit independent of the tree:
+    // every node box contains its leaves' boxes, so it passes whenever the
+    // gate does, and any BVH reaches every gated root.
+    VZ_DEV bool hit_gated(float4 s, float& t) const
+    {
+        if (!hit_sphere(o, d, tmin, best, s, t)) return false;
+        const float r = fabsf(s.w);
+        const float c[3] = {s.x, s.y, s.z}, org[3] = {o.x, o.y, o.z};
+        float lo_t = tmin, hi_t = t;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float t0 = ((c[k] - r) - org[k]) * inv[k];
+            const float t1 = ((c[k] + r) - org[k]) * inv[k];
+            lo_t = fmaxf(lo_t, fminf(t0, t1));
+            hi_t = fminf(hi_t, fmaxf(t0, t1));
+        }
+        return lo_t <= hi_t * kRobust;
+    }
+
     // Sphere test of leaf `leaf`; keeps the lexicographic minimum of (t, prim).
     VZ_DEV void test_leaf(const SceneView& sc, uint32_t leaf, uint32_t& ns)
     {
         float t;
         if (COUNT) ns++;
-        if (hit_sphere(o, d, tmin, best, ldg(sc.spheres + leaf), t)) {
+        if (hit_gated(ldg(sc.spheres + leaf), t)) {
             if (t < best) {
                 best = t;
                 bleaf = leaf;
