@@ -38,7 +38,7 @@ struct ReqDesc {             // device table entry, written by the host
     uint32_t spp, subframe, max_depth, flags;
     uint32_t npix, tw, th, tiles_x, npad;
     uint32_t K;              // pass ring slots
-    uint32_t pad0;
+    uint32_t seq;            // arrival number: free queue room goes to the lowest first
     unsigned long long total_items;  // spp * npad
     float* passbuf;          // K * 3 * npix, planar per slot
     float* sum32;            // 3 * npix
@@ -131,6 +131,7 @@ struct Engine {
     bool stream_alloc = false;
     std::vector<uint8_t> slot_used;
     uint32_t n_slots = 0;
+    uint32_t next_seq = 0;                 // arrival numbers of served requests
     ReqState* h_state_stage = nullptr;     // pinned staging for state uploads
     ReqState* h_status = nullptr;          // pinned, 2 x kMaxRequests status copies
     uint32_t* h_nslots = nullptr;          // pinned staging for the slot count

@@ -304,6 +304,7 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
                                                      uint32_t R)
 {
     __shared__ uint32_t s_scan[1024];
+    __shared__ uint32_t s_quota[1024], s_seq[1024];
     const uint32_t r = threadIdx.x;
     const uint32_t nslots = cnt->n_slots;
     const uint32_t iter = cnt->iteration;
@@ -361,9 +362,20 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         st[r].finish_iter = S.finish_iter;
         st[r].segments = S.segments;
     }
-    uint32_t tq;
-    const uint32_t excl = block_excl_scan(quota, s_scan, &tq);
-    const uint32_t grant = excl >= freeq ? 0u : min(quota, freeq - excl);
+    // free room goes to the requests in arrival order: a request gets what
+    // is left after every earlier (lower seq) request's quota.  Slots are
+    // reused by the serving loop, so slot order is not arrival order.
+    s_quota[r] = quota;
+    s_seq[r] = running ? req[r].seq : 0u;
+    __syncthreads();
+    unsigned long long before = 0;
+    if (quota) {
+        const uint32_t my_seq = s_seq[r];
+        for (uint32_t j = 0; j < nslots; j++)
+            if ((int32_t)(s_seq[j] - my_seq) < 0) before += s_quota[j];  // wrap-safe order
+    }
+    __syncthreads();
+    const uint32_t grant = before >= freeq ? 0u : (uint32_t)min((unsigned long long)quota, freeq - before);
     if (running && grant) st[r].next_item = next_item + grant;
     uint32_t nA, totA, nR, totR, nact;
     const uint32_t posA = block_excl_scan(acc_cnt ? 1u : 0u, s_scan, &nA);
@@ -764,7 +776,8 @@ static vzStatus ensure_graph(Engine* e, bool count, bool lds, bool single)
 }
 
 // Fills slot `slot` of the host table for one request; allocates its buffers.
-static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const vzLaunchParams& p, hipStream_t st)
+static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const vzLaunchParams& p, uint32_t seq,
+                           hipStream_t st)
 {
     ReqDesc& D = e->h_req[slot];
     std::memset(&D, 0, sizeof(D));
@@ -781,6 +794,7 @@ static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const v
     uint32_t tiles_y = (p.height + D.th - 1) / D.th;
     D.npad = D.tiles_x * D.tw * tiles_y * D.th;
     D.K = pass_slots(p);
+    D.seq = seq;
     D.total_items = (unsigned long long)D.spp * D.npad;
     D.accum = p.accum;
     D.image = (uint32_t*)p.image;
@@ -869,7 +883,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
         }
     }
     VZ_TRY(upload_scenes(e, scenes, st));
-    for (uint32_t i = 0; i < n; i++) VZ_TRY(setup_slot(e, i, scene_idx[i], items[i].params, st));
+    for (uint32_t i = 0; i < n; i++) VZ_TRY(setup_slot(e, i, scene_idx[i], items[i].params, i, st));
     VZ_HIP_TRY(hipMemcpyAsync(e->d_req, e->h_req, n * sizeof(ReqDesc), hipMemcpyHostToDevice, st));
     VZ_HIP_TRY(hipMemcpyAsync(e->d_state, e->h_state, n * sizeof(ReqState), hipMemcpyHostToDevice, st));
     Counters c0;
@@ -997,6 +1011,7 @@ vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes)
     e->stream_alloc = true;
     e->slot_used.assign(kMaxRequests, 0);
     e->n_slots = 0;
+    e->next_seq = 0;
     e->scene_list.clear();
     max_scenes = std::max<uint32_t>(max_scenes, 1u);
     if (e->scene_cap < max_scenes) {
@@ -1042,7 +1057,7 @@ vzStatus engine_admit(Engine* e, const RunItem& item, hipStream_t st, uint32_t* 
     } else {
         sidx = (uint32_t)(it - e->scene_list.begin());
     }
-    VZ_TRY(setup_slot(e, slot, sidx, item.params, st));
+    VZ_TRY(setup_slot(e, slot, sidx, item.params, e->next_seq++, st));
     e->h_state_stage[slot] = e->h_state[slot];
     VZ_HIP_TRY(hipMemcpyAsync(e->d_req + slot, e->h_req + slot, sizeof(ReqDesc), hipMemcpyHostToDevice, st));
     VZ_HIP_TRY(hipMemcpyAsync(e->d_state + slot, e->h_state_stage + slot, sizeof(ReqState), hipMemcpyHostToDevice,

@@ -11,9 +11,9 @@
 // the interactive loop that keeps the GAS and accumulator resident across
 // frames (Core/Core.cpp:358-430, Core/Renderer.h:37-54).
 //
-// Autotuner: times the batched form over candidate queue capacities and
-// camera-ray tile shapes on a reduced-sample copy of the caller's batch and
-// keeps the fastest (segments per millisecond), cached per context.
+// Autotuner: times the caller's batch over candidate queue capacities and
+// camera-ray tile shapes and keeps the fastest (segments per millisecond),
+// cached per context.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -298,17 +298,14 @@ vzStatus vzServerDestroy(vzServer srv)
 vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStream stream, vzTuning* out)
 {
     if (!ctx || !items || n == 0 || !out) return set_error(VZ_ERROR_INVALID_VALUE, "NULL argument or empty batch");
-    // reduced-sample copy of the batch: same scenes, sizes and depths, a
-    // quarter of the samples (at least min(spp, 4)) so that each request's
-    // ring of passes in flight stays close to the full batch's
+    // Times the batch itself once per candidate.  The engine's render time
+    // starts after the queue allocation and graph capture, so no warm-up run
+    // is needed.  Reduced-sample probes shrank each request's pass ring and
+    // favoured small queues that lost on the real batch.
     std::vector<vzBatchItem> probe(items, items + n);
-    for (auto& it : probe) {
-        const uint32_t spp = it.params.samples_per_pixel;
-        it.params.samples_per_pixel = std::max(std::min(spp, 4u), spp / 4u);
-        it.params.flags &= ~(uint32_t)(VZ_COUNT_VISITS | VZ_TIME_STAGES);
-    }
-    const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
-    const uint32_t tiles[][2] = {{8, 8}, {16, 4}, {32, 2}, {64, 1}};
+    for (auto& it : probe) it.params.flags &= ~(uint32_t)(VZ_COUNT_VISITS | VZ_TIME_STAGES);
+    const uint32_t caps[] = {1u << 22, 1u << 23, 1u << 24};
+    const uint32_t tiles[][2] = {{32, 2}, {8, 8}};
     vzTuning best{};
     double best_rate = -1.0;
     for (uint32_t cap : caps) {
@@ -320,10 +317,8 @@ vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStrea
             cand.iters_per_graph = 16;
             cand.refill_lanes = 0;
             vzStats st{};
-            // first run warms the engine allocation and graph, second is timed
             VZ_TRY(vzBatchLaunch(ctx, probe.data(), n, &cand, stream, &st));
-            VZ_TRY(vzBatchLaunch(ctx, probe.data(), n, &cand, stream, &st));
-            double rate = st.render_ms > 0.0 ? (double)st.segments / st.render_ms : 0.0;
+            const double rate = st.render_ms > 0.0 ? (double)st.segments / st.render_ms : 0.0;
             if (rate > best_rate) {
                 best_rate = rate;
                 best = cand;
