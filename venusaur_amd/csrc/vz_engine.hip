@@ -277,21 +277,36 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
 }
 
 // ---- control (one workgroup of 1024 threads) -----------------------------------
+// Exclusive scan over the 1024-thread control workgroup: an inclusive scan
+// inside each wave by shuffles, the 16 wave totals scanned by the first wave,
+// two barriers in all.  s needs 16 words.
 __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s, uint32_t* total)
 {
-    const uint32_t t = threadIdx.x;
-    s[t] = v;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024u; off <<= 1) {
-        uint32_t x = t >= off ? s[t - off] : 0u;
-        __syncthreads();
-        s[t] += x;
-        __syncthreads();
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const uint32_t x = __shfl_up(incl, off);
+        if (lane >= off) incl += x;
     }
-    uint32_t incl = s[t];
-    *total = s[1023];
+    if (lane == 63u) s[wave] = incl;
     __syncthreads();
-    return incl - v;
+    if (wave == 0) {
+        const uint32_t w = lane < 16u ? s[lane] : 0u;
+        uint32_t wi = w;
+#pragma unroll
+        for (uint32_t off = 1; off < 16u; off <<= 1) {
+            const uint32_t x = __shfl_up(wi, off);
+            if (lane >= off) wi += x;
+        }
+        if (lane < 16u) s[16 + lane] = wi - w;  // exclusive prefix of the wave totals
+        if (lane == 15u) s[32] = wi;
+    }
+    __syncthreads();
+    const uint32_t r = s[16 + wave] + incl - v;
+    *total = s[32];
+    __syncthreads();  // s is reused by the next scan
+    return r;
 }
 
 constexpr uint32_t kSlicesPerThread = kMaxBlocks / 1024;
