@@ -66,7 +66,7 @@ struct Counters {            // written by the control kernel only
     uint32_t total_regen;
 };
 
-struct BlockStats {          // one per step/regen workgroup, single writer
+struct BlockStats {          // one per queue slice: its step workgroup writes hits..segments, the regen workgroups add paths atomically
     unsigned long long hits, node_visits, sphere_tests, paths, segments;
 };
 

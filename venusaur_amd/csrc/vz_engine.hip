@@ -24,6 +24,7 @@ constexpr uint32_t kLdsNodes = 512;  // scenes up to this many nodes are staged 
 constexpr uint32_t kRefillLds = 1;
 constexpr uint32_t kRefillGlobal = 41;
 constexpr int kRegenBlk = 256;
+constexpr uint32_t kRegenOver = 8;  // regen workgroups per queue slice: ~one pixel run per thread at 1080p
 constexpr int kBlk = 256;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
@@ -613,7 +614,8 @@ __global__ void __launch_bounds__(kRegenBlk) k_wf_regen(const ReqDesc* __restric
     if (tid == 0) {
         unsigned long long p = 0;
         for (int w = 0; w < kRegenBlk / 64; w++) p += s_paths[w];
-        bstats[blockIdx.x].paths += p;
+        // kRegenOver regen workgroups share each slice's stats entry
+        if (p) atomicAdd(&bstats[blockIdx.x % NB].paths, p);
     }
 }
 
@@ -765,7 +767,7 @@ static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool
     hipLaunchKernelGGL(k_wf_control, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt, e->d_plan,
                        cnt_nxt, e->NB, e->R);
     hipLaunchKernelGGL(k_wf_apply, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_req, e->d_plan);
-    hipLaunchKernelGGL(k_wf_regen, dim3(e->NB), dim3(kRegenBlk), 0, st, e->d_req, e->d_plan, e->NB, e->q_o[n],
+    hipLaunchKernelGGL(k_wf_regen, dim3(kRegenOver * e->NB), dim3(kRegenBlk), 0, st, e->d_req, e->d_plan, e->NB, e->q_o[n],
                        e->q_d[n], e->q_t[n], e->d_bstats, e->R);
 }
 
