@@ -57,10 +57,17 @@ def _sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
+def _progress(msg):
+    """Progress of long phases on stderr (stdout carries the one JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _build_scenes(ctx, reqs):
     from venusaur_amd import api
     scenes, build_ms = {}, []
-    for r in reqs:
+    for k, r in enumerate(reqs):
+        if k % 200 == 0:
+            _progress(f"building scenes: request {k}/{len(reqs)}")
         if r.scene_key not in scenes:
             s, m = api.box_scene(r.n_small, 1000 + r.scene_key)
             scenes[r.scene_key] = api.Scene(ctx, s, m)
@@ -205,7 +212,9 @@ def run_served(ctx, n_requests, rmse_samples=3):
     # plain: one request at a time in arrival order
     lat_plain, segs = [], 0
     t_start = time.perf_counter()
-    for r in reqs:
+    for k, r in enumerate(reqs):
+        if k % 100 == 0:
+            _progress(f"served comparison, plain launches: {k}/{len(reqs)}")
         now = time.perf_counter() - t_start
         if now < r.arrival_s:
             time.sleep(r.arrival_s - now)
@@ -228,11 +237,18 @@ def run_served(ctx, n_requests, rmse_samples=3):
     th = threading.Thread(target=client)
     th.start()
     th.join()
+    while True:  # poll until every request has finished, reporting progress
+        res = [srv.poll(i) for i in ids]
+        left = sum(x is None for x in res)
+        if left == 0:
+            break
+        _progress(f"served comparison, server: {len(ids) - left}/{len(ids)} requests done")
+        time.sleep(10.0)
     srv.drain()
-    res = [srv.poll(i) for i in ids]
     off = t_start - t_srv0
     lat_srv = [(x.finish_ms / 1e3 - off) - r.arrival_s for x, r in zip(res, reqs)]
     t_srv_end = max(x.finish_ms for x in res) / 1e3 - off
+    _progress("served comparison: checking results against the plain launches")
     same = all(_sha(r.accum_host()) == w for r, w in zip(reqs, want))
     ok = all(x.status == 0 for x in res)
     srv.close()
@@ -286,6 +302,7 @@ def main():
     p.accum, p.image = acc.value, img.value
 
     # untimed: warmup, then one counting pass for the algorithmic bytes
+    _progress(f"book scene {W}x{H} {SPP} spp: warm-up, counting frame, {a.steps} timed frames")
     for _ in range(a.warmup):
         scene.launch(p)
     pc = api.LaunchParams.from_buffer_copy(p)
@@ -339,6 +356,7 @@ def main():
     checks = {"gpu_segments_per_step": stats[0].segments, "paths_per_step": stats[0].paths,
               "lbvh_build_ms": round(build_ms, 3)}
     if not a.no_cpu:
+        _progress("CPU baseline (oracle) on sampled rows")
         import oracle as O
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         threads = min(threads, os.cpu_count() or threads)
@@ -369,10 +387,13 @@ def main():
 
     forms = {}
     if a.c2_frames:
+        _progress("traversal-bound scene (1M spheres)")
         forms["traversal_bound"] = run_c2(ctx, a.c2_frames)
     if a.batch_scenes:
+        _progress(f"batched form ({a.batch_scenes} scenes)")
         forms["batched"] = run_batched(ctx, a.batch_scenes)
     if a.serve_requests:
+        _progress(f"served form ({a.serve_requests} requests)")
         forms["served"] = run_served(ctx, a.serve_requests)
 
     info = ctx.info()

@@ -21,7 +21,7 @@ constexpr uint32_t kLdsNodes = 512;  // scenes up to this many nodes are staged 
 // trace (1 = only when all are done, 65 = after every node step).  Defaults per
 // node source, chosen by sweeping the 1080p book scene (LDS nodes) and
 // 1M-sphere box scenes (global nodes).
-constexpr uint32_t kRefillLds = 1;
+constexpr uint32_t kRefillLds = 9;
 constexpr uint32_t kRefillGlobal = 41;
 constexpr int kRegenBlk = 256;
 constexpr uint32_t kRegenOver = 8;  // regen workgroups per queue slice: ~one pixel run per thread at 1080p
