@@ -61,6 +61,7 @@ template <int S, bool COUNT>
 struct Traversal {
     v3 o, d;
     float inv[3];
+    uint32_t sel[3];  // per axis: byte selector that puts the near plane first
     float tmin, best;
     uint32_t bleaf, bprim, node, sp, bottom, budget;
 
@@ -79,6 +80,10 @@ struct Traversal {
             float dk = dd[k];
             if (!(fabsf(dk) > 1e-20f)) dk = copysignf(1e-20f, dk);
             inv[k] = 1.0f / dk;
+            // lo <= hi and rounding is monotone, so (lo - o) * inv <= (hi - o) * inv
+            // exactly when inv > 0: the near plane of each child is its lo
+            // byte for a positive direction, its hi byte for a negative one
+            sel[k] = inv[k] < 0.0f ? 0x02030001u : 0x03020100u;
         }
         if (sc.n == 1u) {
             float t;
@@ -160,18 +165,22 @@ struct Traversal {
         const uint32_t g = B.w;
         float tn[2];
         bool hit[2];
+        // per axis, the two children's (lo, hi) bytes reordered to (near, far):
+        // one byte permute replaces a min and a max per child and axis, with
+        // the same result bit for bit
+        uint32_t qs[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) qs[k] = __builtin_amdgcn_perm(qw[k], qw[k], sel[k]);
 #pragma unroll
         for (int c = 0; c < 2; c++) {
             float lo_t = tmin, hi_t = best;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                float s = scale_of(meta, k);
-                float lo = __builtin_fmaf(s, (float)((qw[k] >> (16 * c)) & 0xFFu), no[k]);
-                float hi = __builtin_fmaf(s, (float)((qw[k] >> (16 * c + 8)) & 0xFFu), no[k]);
-                float t0 = (lo - org[k]) * inv[k];
-                float t1 = (hi - org[k]) * inv[k];
-                lo_t = fmaxf(lo_t, fminf(t0, t1));
-                hi_t = fminf(hi_t, fmaxf(t0, t1));
+                const float s = scale_of(meta, k);
+                const float pn = __builtin_fmaf(s, (float)((qs[k] >> (16 * c)) & 0xFFu), no[k]);
+                const float pf = __builtin_fmaf(s, (float)((qs[k] >> (16 * c + 8)) & 0xFFu), no[k]);
+                lo_t = fmaxf(lo_t, (pn - org[k]) * inv[k]);
+                hi_t = fminf(hi_t, (pf - org[k]) * inv[k]);
             }
             tn[c] = lo_t;
             hit[c] = lo_t <= hi_t * kRobust;
