@@ -113,12 +113,20 @@ vzStatus vzDeviceInfo(vzContext ctx, uint32_t* num_cus, uint64_t* hbm_bytes, cha
  * LBVH on the GPU (replaces Renderer.h:160-255 and 452-520). */
 vzStatus vzSceneCreate(vzContext ctx, const vzSphere* spheres, const vzMaterial* materials,
                        uint32_t n, vzStream stream, vzScene* out);
+/* As vzSceneCreate, with spheres and materials already in device memory
+ * (copied device to device and validated on the GPU; synchronises the
+ * stream).  For callers that generate or keep scenes in HBM. */
+vzStatus vzSceneCreateDevice(vzContext ctx, const vzSphere* spheres, const vzMaterial* materials,
+                             uint32_t n, vzStream stream, vzScene* out);
 vzStatus vzSceneDestroy(vzScene scene);
 /* Rebuild the LBVH from the current spheres. */
 vzStatus vzAccelBuild(vzScene scene, vzStream stream);
 /* Replace sphere geometry (same count, same materials) and refit the
  * existing topology: boxes and node quantisation are recomputed bottom-up. */
 vzStatus vzSceneUpdateSpheres(vzScene scene, const vzSphere* spheres, vzStream stream);
+/* As vzSceneUpdateSpheres, with the spheres in device memory (validated on
+ * the GPU; a rejected update leaves the scene unchanged). */
+vzStatus vzSceneUpdateSpheresDevice(vzScene scene, const vzSphere* spheres, vzStream stream);
 vzStatus vzAccelRefit(vzScene scene, vzStream stream);
 vzStatus vzAccelInfoGet(vzScene scene, vzAccelInfo* out);
 /* Copies the BVH to host memory for inspection (synchronises the stream).

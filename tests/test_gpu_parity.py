@@ -217,6 +217,40 @@ def test_refit_after_motion(ctx):
     compare_bvh(sc, s2)
 
 
+def test_scene_from_device_pointers(ctx):
+    say("case: scene created and updated from device pointers")
+    s, m = api.box_scene(5000, 31)
+    ds, dm = DeviceBuffer.from_array(s), DeviceBuffer.from_array(m)
+    a = api.Scene(ctx, s, m)
+    b = api.Scene.from_device(ctx, ds.value, dm.value, len(s))
+    for x, y in zip(a.download(), b.download()):
+        assert x.tobytes() == y.tobytes()
+    # (the device buffers stay referenced until the calls that read them return)
+    s2 = s.copy()
+    s2["cx"][1:] += np.float32(0.125)
+    ds2 = DeviceBuffer.from_array(s2)
+    a.update_spheres(s2)
+    b.update_spheres_device(ds2.value)
+    for x, y in zip(a.download(), b.download()):
+        assert x.tobytes() == y.tobytes()
+    # invalid device input is rejected with the offending index; a rejected
+    # update leaves the scene as it was
+    bad = s2.copy()
+    bad["r"][17] = 0.0
+    dbad = DeviceBuffer.from_array(bad)
+    with pytest.raises(api.VzError, match="sphere 17"):
+        b.update_spheres_device(dbad.value)
+    for x, y in zip(a.download(), b.download()):
+        assert x.tobytes() == y.tobytes()
+    with pytest.raises(api.VzError, match="sphere 17"):
+        api.Scene.from_device(ctx, dbad.value, dm.value, len(s))
+    badm = m.copy()
+    badm["type"][5] = 9
+    dbadm = DeviceBuffer.from_array(badm)
+    with pytest.raises(api.VzError, match="material 5"):
+        api.Scene.from_device(ctx, ds.value, dbadm.value, len(s))
+
+
 def test_invalid_params_raise(ctx, book):
     say("case: invalid params")
     s, m = book

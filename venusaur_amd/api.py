@@ -92,6 +92,8 @@ def _declare(L):
         "vzDestroy": (st, [vp]),
         "vzDeviceInfo": (st, [vp, P(u32), P(u64), C.c_char_p, C.c_size_t]),
         "vzSceneCreate": (st, [vp, vp, vp, u32, vp, P(vp)]),
+        "vzSceneCreateDevice": (st, [vp, vp, vp, u32, vp, P(vp)]),
+        "vzSceneUpdateSpheresDevice": (st, [vp, vp, vp]),
         "vzSceneDestroy": (st, [vp]),
         "vzAccelBuild": (st, [vp, vp]),
         "vzSceneUpdateSpheres": (st, [vp, vp, vp]),
@@ -214,9 +216,22 @@ class Scene:
         self.ctx = ctx
         self.spheres = np.ascontiguousarray(spheres, SPHERE_DT)
         self.materials = np.ascontiguousarray(materials, MATERIAL_DT)
+        self.n = len(self.spheres)
         self.h = C.c_void_p()
-        check(lib().vzSceneCreate(ctx.h, _p(self.spheres), _p(self.materials), len(self.spheres),
+        check(lib().vzSceneCreate(ctx.h, _p(self.spheres), _p(self.materials), self.n,
                                   stream, C.byref(self.h)))
+
+    @classmethod
+    def from_device(cls, ctx, spheres_dev_ptr, materials_dev_ptr, n, stream=None):
+        """vzSceneCreateDevice: spheres and materials already in device memory."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.spheres = self.materials = None
+        self.n = int(n)
+        self.h = C.c_void_p()
+        check(lib().vzSceneCreateDevice(ctx.h, spheres_dev_ptr, materials_dev_ptr, self.n, stream,
+                                        C.byref(self.h)))
+        return self
 
     def close(self):
         if self.h:
@@ -238,11 +253,17 @@ class Scene:
         check(lib().vzAccelBuild(self.h, stream))
 
     def update_spheres(self, spheres, stream=None):
-        self.spheres = np.ascontiguousarray(spheres, SPHERE_DT)
-        check(lib().vzSceneUpdateSpheres(self.h, _p(self.spheres), stream))
+        spheres = np.ascontiguousarray(spheres, SPHERE_DT)
+        check(lib().vzSceneUpdateSpheres(self.h, _p(spheres), stream))
+        self.spheres = spheres
+
+    def update_spheres_device(self, spheres_dev_ptr, stream=None):
+        """vzSceneUpdateSpheresDevice: the new spheres are in device memory."""
+        check(lib().vzSceneUpdateSpheresDevice(self.h, spheres_dev_ptr, stream))
+        self.spheres = None
 
     def download(self, stream=None):
-        n = len(self.spheres)
+        n = self.n
         nodes = np.zeros(max(n - 1, 0), NODE_DT)
         leaf = np.zeros(n, np.uint32)
         morton = np.zeros(n, np.uint32)

@@ -173,6 +173,40 @@ vzStatus vzSceneCreate(vzContext ctx, const vzSphere* spheres, const vzMaterial*
     return VZ_SUCCESS;
 }
 
+vzStatus vzSceneCreateDevice(vzContext ctx, const vzSphere* spheres, const vzMaterial* materials, uint32_t n,
+                             vzStream stream, vzScene* out)
+{
+    if (!ctx || !spheres || !materials || !out) return set_error(VZ_ERROR_INVALID_VALUE, "NULL argument");
+    *out = nullptr;
+    if (n == 0 || n >= (1u << 30)) return set_error(VZ_ERROR_INVALID_VALUE, "sphere count must be in [1, 2^30)");
+    VZ_HIP_TRY(hipSetDevice(ctx->device));
+    vzScene s = new vzScene_st();
+    s->ctx = ctx;
+    s->n = n;
+    vzStatus rs = lbvh_alloc(s);
+    hipStream_t st = as_stream(stream);
+    if (rs == VZ_SUCCESS &&
+        (hipMemcpyAsync(s->d_sph_in, spheres, sizeof(vzSphere) * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+         hipMemcpyAsync(s->d_mat_in, materials, sizeof(vzMaterial) * n, hipMemcpyDeviceToDevice, st) != hipSuccess))
+        rs = set_error(VZ_ERROR_HIP, "scene copy failed");
+    uint32_t bad_s = 0xFFFFFFFFu, bad_m = 0xFFFFFFFFu;
+    if (rs == VZ_SUCCESS) rs = lbvh_check_input(s, s->d_mat_in, st, &bad_s, &bad_m);
+    if (rs == VZ_SUCCESS && bad_s != 0xFFFFFFFFu)
+        rs = set_error(VZ_ERROR_INVALID_VALUE,
+                       "sphere " + std::to_string(bad_s) + " has a non-finite centre or a zero/non-finite radius");
+    if (rs == VZ_SUCCESS && bad_m != 0xFFFFFFFFu)
+        rs = set_error(VZ_ERROR_INVALID_VALUE, "material " + std::to_string(bad_m) + " has an unknown type");
+    if (rs == VZ_SUCCESS) rs = lbvh_build(s, st);
+    if (rs != VZ_SUCCESS) {
+        (void)hipStreamSynchronize(st);
+        lbvh_free(s);
+        delete s;
+        return rs;
+    }
+    *out = s;
+    return VZ_SUCCESS;
+}
+
 vzStatus vzSceneDestroy(vzScene scene)
 {
     if (!scene) return set_error(VZ_ERROR_INVALID_VALUE, "scene is NULL");
@@ -196,6 +230,34 @@ vzStatus vzSceneUpdateSpheres(vzScene scene, const vzSphere* spheres, vzStream s
     VZ_HIP_TRY(hipMemcpyAsync(scene->d_sph_in, spheres, sizeof(vzSphere) * scene->n, hipMemcpyHostToDevice,
                               as_stream(stream)));
     return lbvh_refit(scene, as_stream(stream));
+}
+
+vzStatus vzSceneUpdateSpheresDevice(vzScene scene, const vzSphere* spheres, vzStream stream)
+{
+    if (!scene || !spheres) return set_error(VZ_ERROR_INVALID_VALUE, "NULL argument");
+    hipStream_t st = as_stream(stream);
+    // check a copy first, so a rejected update leaves the scene unchanged
+    vzSphere* tmp = nullptr;
+    VZ_HIP_TRY(hipMallocAsync((void**)&tmp, sizeof(vzSphere) * scene->n, st));
+    vzStatus rs = VZ_SUCCESS;
+    if (hipMemcpyAsync(tmp, spheres, sizeof(vzSphere) * scene->n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        rs = set_error(VZ_ERROR_HIP, "sphere copy failed");
+    float4* keep = scene->d_sph_in;
+    uint32_t bad_s = 0xFFFFFFFFu, bad_m = 0xFFFFFFFFu;
+    if (rs == VZ_SUCCESS) {
+        scene->d_sph_in = reinterpret_cast<float4*>(tmp);
+        rs = lbvh_check_input(scene, nullptr, st, &bad_s, &bad_m);
+        scene->d_sph_in = keep;
+    }
+    if (rs == VZ_SUCCESS && bad_s != 0xFFFFFFFFu)
+        rs = set_error(VZ_ERROR_INVALID_VALUE,
+                       "sphere " + std::to_string(bad_s) + " has a non-finite centre or a zero/non-finite radius");
+    if (rs == VZ_SUCCESS &&
+        hipMemcpyAsync(scene->d_sph_in, tmp, sizeof(vzSphere) * scene->n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        rs = set_error(VZ_ERROR_HIP, "sphere copy failed");
+    (void)hipFreeAsync(tmp, st);
+    VZ_TRY(rs);
+    return lbvh_refit(scene, st);
 }
 
 vzStatus vzAccelRefit(vzScene scene, vzStream stream)

@@ -107,6 +107,11 @@ namespace vz {
 vzStatus lbvh_alloc(vzScene s);
 vzStatus lbvh_build(vzScene s, hipStream_t st);   // full build from d_sph_in
 vzStatus lbvh_refit(vzScene s, hipStream_t st);   // same topology, new d_sph_in
+// Checks d_sph_in (and `mat`, device memory, when not NULL) on the GPU;
+// returns the first bad sphere / material index, 0xFFFFFFFF for none.
+// Synchronises the stream.
+vzStatus lbvh_check_input(vzScene s, const vzMaterial* mat, hipStream_t st, uint32_t* bad_sphere,
+                          uint32_t* bad_material);
 void lbvh_free(vzScene s);
 
 // vz_trace.hip

@@ -571,6 +571,27 @@ __global__ void k_zero(uint32_t* a, uint32_t n)
     if (i < n) a[i] = 0u;
 }
 
+__global__ void k_fill(uint32_t* a, uint32_t n, uint32_t v)
+{
+    uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// Validation of device-resident input: first[0] = lowest index of a sphere
+// with a non-finite centre or a zero/non-finite radius, first[1] = lowest
+// index of a material with an unknown type (0xFFFFFFFF: none).
+__global__ void __launch_bounds__(kBlk) k_check_input(const float4* __restrict__ sph,
+                                                      const vzMaterial* __restrict__ mat, uint32_t n,
+                                                      uint32_t* __restrict__ first)
+{
+    for (uint32_t i = blockIdx.x * kBlk + threadIdx.x; i < n; i += gridDim.x * kBlk) {
+        const float4 c = sph[i];
+        if (!(isfinite(c.x) && isfinite(c.y) && isfinite(c.z) && isfinite(c.w) && c.w != 0.0f))
+            atomicMin(&first[0], i);
+        if (mat && mat[i].type > VZ_DIELECTRIC) atomicMin(&first[1], i);
+    }
+}
+
 inline uint32_t blocks_for(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
 
 }  // namespace
@@ -690,6 +711,22 @@ vzStatus lbvh_build(vzScene s, hipStream_t st)
     (void)hipEventDestroy(e1);
     if (n < 2) s->depth = 1;
     return rs;
+}
+
+vzStatus lbvh_check_input(vzScene s, const vzMaterial* mat, hipStream_t st, uint32_t* bad_sphere,
+                          uint32_t* bad_material)
+{
+    uint32_t* first = s->d_fcount + 2;  // words 2-3 of the refit counter block
+    uint32_t h[2] = {0u, 0u};
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlk), 0, st, first, 2u, 0xFFFFFFFFu);
+    const uint32_t nb = std::min<uint32_t>(1024u, blocks_for(s->n, kBlk));
+    hipLaunchKernelGGL(k_check_input, dim3(nb), dim3(kBlk), 0, st, s->d_sph_in, mat, s->n, first);
+    VZ_HIP_TRY(hipGetLastError());
+    VZ_HIP_TRY(hipMemcpyAsync(h, first, sizeof(h), hipMemcpyDeviceToHost, st));
+    VZ_HIP_TRY(hipStreamSynchronize(st));
+    *bad_sphere = h[0];
+    *bad_material = h[1];
+    return VZ_SUCCESS;
 }
 
 vzStatus lbvh_refit(vzScene s, hipStream_t st)
