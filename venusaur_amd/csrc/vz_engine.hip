@@ -148,7 +148,9 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
             if (exhausted) break;
             continue;  // claimed only padding items
         }
-        // trace stage: advance every busy lane until enough of them finish
+        // trace stage: advance every busy lane until enough of them finish.
+        // (Parking lanes whose node has leaf children, to run the sphere tests
+        // of many lanes together, was measured slower at every threshold.)
         if (refill <= 1u) {  // the whole wave finishes: a plain divergent loop
             while (active && !tdone) {
                 if (LDSN)

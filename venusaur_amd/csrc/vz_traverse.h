@@ -64,6 +64,8 @@ struct Traversal {
     uint32_t sel[3];  // per axis: byte selector that puts the near plane first
     float tmin, best;
     uint32_t bleaf, bprim, node, sp, bottom, budget;
+    uint32_t pend, pg;  // the visit waiting for leaves() / decide() (see visit())
+    float ptn0, ptn1;
 
     // Returns true when the ray is already finished (single-sphere scene).
     VZ_DEV bool init(const SceneView& sc, v3 o_, v3 d_, float tmin_, float tmax_, uint32_t& ns)
@@ -101,6 +103,7 @@ struct Traversal {
         // once; this and the index guards below only matter for a corrupted
         // node array
         budget = sc.n;
+        pend = 0u;
         return false;
     }
 
@@ -150,10 +153,19 @@ struct Traversal {
         }
     }
 
-    // Visits one node; returns true when the traversal has finished.
+    // A node visit in three parts, so that a wave can run the sphere tests of
+    // many lanes together (Aila & Laine 2009, "while-while"): visit() does the
+    // box tests and records what the node leads to; leaves() tests its leaf
+    // children; decide() picks the next node with the best hit as it stands
+    // after those tests.  Each lane runs the same sequence as step(), so the
+    // result and every count are the same whenever leaves() runs.
+    //
+    // visit(): returns true when the traversal has finished (a guard fired).
+    // Otherwise pend holds the leaf children to test (bits 0-1) and the
+    // internal children to follow (bits 2-3); pg is the node's first child
+    // and ptn the two children's entry distances.
     template <class NodeSrc>
-    VZ_DEV bool step(const SceneView& sc, const NodeSrc& nodes, uint32_t* lds_node, float* lds_t,
-                     uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns)
+    VZ_DEV bool visit(const SceneView& sc, const NodeSrc& nodes, uint32_t& nv)
     {
         uint4 A, B;
         nodes.fetch(node, A, B);
@@ -189,18 +201,36 @@ struct Traversal {
         // together: children must be valid indices, and a well-formed tree
         // visits each of its n-1 internal nodes at most once
         if (g + 1u >= sc.n || budget-- == 0u) return true;
-        const uint32_t flags = meta >> 24;
-        // leaf children, in child order
-#pragma unroll
-        for (int c = 0; c < 2; c++)
-            if (hit[c] && ((flags >> c) & 1u)) test_leaf(sc, g + (uint32_t)c, ns);
-        const bool in0 = hit[0] && !(flags & 1u);
-        const bool in1 = hit[1] && !(flags & 2u);
+        const uint32_t leaf = (meta >> 24) & 3u;
+        const uint32_t h = (hit[0] ? 1u : 0u) | (hit[1] ? 2u : 0u);
+        pend = (h & leaf) | ((h & ~leaf & 3u) << 2);
+        pg = g;
+        ptn0 = tn[0];
+        ptn1 = tn[1];
+        return false;
+    }
+    VZ_DEV bool has_leaves() const { return (pend & 3u) != 0u; }
+
+    // Sphere tests of the pending visit's leaf children, in child order.
+    VZ_DEV void leaves(const SceneView& sc, uint32_t& ns)
+    {
+        if (pend & 1u) test_leaf(sc, pg, ns);
+        if (pend & 2u) test_leaf(sc, pg + 1u, ns);
+    }
+
+    // Next node after the pending visit: the near internal child (the far one
+    // pushed), else the only one, else the nearest stacked node that can
+    // still hold a closer hit.  Returns true when the traversal has finished.
+    VZ_DEV bool decide(const SceneView& sc, uint32_t* lds_node, float* lds_t, uint32_t stride, Overflow& ov)
+    {
+        const bool in0 = (pend & 4u) != 0u, in1 = (pend & 8u) != 0u;
+        const uint32_t g = pg;
+        pend = 0u;
         if (in1 && g + 2u >= sc.n) return true;  // internal child g+1 must be <= n-2
         if (in0 && in1) {
-            const int nr = (tn[1] < tn[0]) ? 1 : 0;
+            const int nr = (ptn1 < ptn0) ? 1 : 0;
             const uint32_t far_node = g + (uint32_t)(1 - nr);
-            const float far_t = tn[1 - nr];
+            const float far_t = nr ? ptn0 : ptn1;
             if (sp - bottom == (uint32_t)S) {  // ring full: oldest entry to the overflow
                 if (bottom >= kMaxDepth) return true;
                 ov.node[bottom] = lds_node[(bottom & (S - 1)) * stride];
@@ -237,6 +267,16 @@ struct Traversal {
             }
         }
         return true;
+    }
+
+    // One whole node visit; returns true when the traversal has finished.
+    template <class NodeSrc>
+    VZ_DEV bool step(const SceneView& sc, const NodeSrc& nodes, uint32_t* lds_node, float* lds_t,
+                     uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns)
+    {
+        if (visit(sc, nodes, nv)) return true;
+        leaves(sc, ns);
+        return decide(sc, lds_node, lds_t, stride, ov);
     }
 };
 
