@@ -103,6 +103,8 @@ typedef struct vzAccelInfo {
 
 /* ---- context -------------------------------------------------------- */
 vzStatus vzCreate(int device, vzContext* out);
+/* Fails with VZ_ERROR_INVALID_VALUE while scenes or servers of the context
+ * are alive: destroy them first. */
 vzStatus vzDestroy(vzContext ctx);
 /* Device properties used by the autotuner/admission: CU count, HBM bytes. */
 vzStatus vzDeviceInfo(vzContext ctx, uint32_t* num_cus, uint64_t* hbm_bytes, char* name,

@@ -284,3 +284,23 @@ def test_c0_error_vs_fp64_oracle(ctx, book):
     e64 = rmse(got, want64)
     say(f"RMSE vs FP64 oracle at 16 spp: {e64:.3e}")
     assert e64 < 0.05  # statistical: paths diverge chaotically, stated not bounded at 1e-3
+
+
+def test_context_outlives_its_scenes_and_servers():
+    say("case: context destroy with live scenes / servers")
+    c = api.Context(0)
+    s, m = api.box_scene(200, 3)
+    sc = api.Scene(c, s, m)
+    srv = api.Server(c, hbm_budget_bytes=1 << 30, max_active=4)
+    st = api.lib().vzDestroy(c.h)
+    assert api.STATUS_NAMES[st] == "VZ_ERROR_INVALID_VALUE"
+    assert "1 scene(s) and 1 server(s)" in api.lib().vzGetLastError().decode()
+    srv.close()
+    sc.close()
+    c.close()  # now succeeds
+    # the Python context closes what it still owns before destroying itself
+    c2 = api.Context(0)
+    api.Scene(c2, s, m)
+    keep = api.Scene(c2, s, m)
+    c2.close()
+    assert not keep.h

@@ -6,6 +6,7 @@ missing or no gfx950 device is visible, calls raise VzError.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -184,11 +185,14 @@ class Context:
 
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self._children = weakref.WeakSet()  # scenes and servers, closed before the context
         check(lib().vzCreate(device, C.byref(self.h)))
 
     def close(self):
         if self.h:
-            lib().vzDestroy(self.h)
+            for ch in list(self._children):
+                ch.close()
+            check(lib().vzDestroy(self.h))
             self.h = C.c_void_p()
 
     def __del__(self):
@@ -220,6 +224,7 @@ class Scene:
         self.h = C.c_void_p()
         check(lib().vzSceneCreate(ctx.h, _p(self.spheres), _p(self.materials), self.n,
                                   stream, C.byref(self.h)))
+        ctx._children.add(self)
 
     @classmethod
     def from_device(cls, ctx, spheres_dev_ptr, materials_dev_ptr, n, stream=None):
@@ -231,6 +236,7 @@ class Scene:
         self.h = C.c_void_p()
         check(lib().vzSceneCreateDevice(ctx.h, spheres_dev_ptr, materials_dev_ptr, self.n, stream,
                                         C.byref(self.h)))
+        ctx._children.add(self)
         return self
 
     def close(self):
@@ -317,6 +323,7 @@ class Server:
             cfg.tuning = tuning
         self.h = C.c_void_p()
         check(lib().vzServerCreate(ctx.h, C.byref(cfg), C.byref(self.h)))
+        ctx._children.add(self)
 
     def submit(self, scene, params):
         rid = C.c_uint64()

@@ -34,7 +34,7 @@ static vzStatus check_spheres(const vzSphere* s, uint32_t n)
     return VZ_SUCCESS;
 }
 
-static vzStatus check_params(const vzLaunchParams* p)
+vzStatus check_launch_params(const vzLaunchParams* p)
 {
     if (!p) return set_error(VZ_ERROR_INVALID_VALUE, "params is NULL");
     if (p->width < 2 || p->height < 2)
@@ -125,6 +125,10 @@ vzStatus vzCreate(int device, vzContext* out)
 vzStatus vzDestroy(vzContext ctx)
 {
     if (!ctx) return set_error(VZ_ERROR_INVALID_VALUE, "ctx is NULL");
+    const uint32_t ns = ctx->live_scenes.load(), nsrv = ctx->live_servers.load();
+    if (ns || nsrv)
+        return set_error(VZ_ERROR_INVALID_VALUE, std::to_string(ns) + " scene(s) and " + std::to_string(nsrv) +
+                                                     " server(s) of this context are still alive");
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     if (ctx->engine) engine_destroy(ctx->engine);
@@ -165,10 +169,12 @@ vzStatus vzSceneCreate(vzContext ctx, const vzSphere* spheres, const vzMaterial*
         rs = set_error(VZ_ERROR_HIP, "scene upload failed");
     if (rs == VZ_SUCCESS) rs = lbvh_build(s, st);
     if (rs != VZ_SUCCESS) {
+        (void)hipStreamSynchronize(st);
         lbvh_free(s);
         delete s;
         return rs;
     }
+    ctx->live_scenes++;
     *out = s;
     return VZ_SUCCESS;
 }
@@ -203,6 +209,7 @@ vzStatus vzSceneCreateDevice(vzContext ctx, const vzSphere* spheres, const vzMat
         delete s;
         return rs;
     }
+    ctx->live_scenes++;
     *out = s;
     return VZ_SUCCESS;
 }
@@ -212,6 +219,7 @@ vzStatus vzSceneDestroy(vzScene scene)
     if (!scene) return set_error(VZ_ERROR_INVALID_VALUE, "scene is NULL");
     (void)hipSetDevice(scene->ctx->device);
     (void)hipDeviceSynchronize();
+    scene->ctx->live_scenes--;
     lbvh_free(scene);
     delete scene;
     return VZ_SUCCESS;
@@ -297,20 +305,23 @@ vzStatus vzTraceClosest(vzScene scene, const vzRay* rays, uint32_t n, vzHit* hit
     DevCounters c;
     VZ_HIP_TRY(hipMalloc(&c.d, 8 * sizeof(unsigned long long)));
     VZ_HIP_TRY(hipMemsetAsync(c.d, 0, 8 * sizeof(unsigned long long), st));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Event e0, e1;
+    VZ_EVENT_TRY(e0);
+    VZ_EVENT_TRY(e1);
+    VZ_HIP_TRY(hipEventRecord(e0.e, st));
+    vzStatus rs = trace_rays(scene->view(), rays, n, hits, st, stats != nullptr, c.d);
+    unsigned long long h[8] = {};
+    if (rs == VZ_SUCCESS && hipEventRecord(e1.e, st) != hipSuccess)
+        rs = set_error(VZ_ERROR_HIP, "hipEventRecord failed");
+    if (rs == VZ_SUCCESS && stats && hipMemcpyAsync(h, c.d, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess)
+        rs = set_error(VZ_ERROR_HIP, "hipMemcpyAsync of the trace counters failed");
+    // c.d is freed on return: the kernel must be done with it, on every path
+    if (hipStreamSynchronize(st) != hipSuccess && rs == VZ_SUCCESS)
+        rs = set_error(VZ_ERROR_HIP, "hipStreamSynchronize failed");
+    VZ_TRY(rs);
     if (stats) {
-        VZ_HIP_TRY(hipEventCreate(&e0));
-        VZ_HIP_TRY(hipEventCreate(&e1));
-        VZ_HIP_TRY(hipEventRecord(e0, st));
-    }
-    VZ_TRY(trace_rays(scene->view(), rays, n, hits, st, stats != nullptr, c.d));
-    if (stats) {
-        VZ_HIP_TRY(hipEventRecord(e1, st));
-        unsigned long long h[8];
-        VZ_HIP_TRY(hipMemcpyAsync(h, c.d, sizeof(h), hipMemcpyDeviceToHost, st));
-        VZ_HIP_TRY(hipStreamSynchronize(st));
         float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0.e, e1.e);
         std::memset(stats, 0, sizeof(*stats));
         stats->segments = h[0];
         stats->node_visits = h[2];
@@ -318,11 +329,6 @@ vzStatus vzTraceClosest(vzScene scene, const vzRay* rays, uint32_t n, vzHit* hit
         stats->render_ms = ms;
         stats->trace_ms = ms;
         stats->trace_launches = 1;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-    } else {
-        // c.d is freed on return: the kernel must be done with it
-        VZ_HIP_TRY(hipStreamSynchronize(st));
     }
     return VZ_SUCCESS;
 }
@@ -330,7 +336,7 @@ vzStatus vzTraceClosest(vzScene scene, const vzRay* rays, uint32_t n, vzHit* hit
 vzStatus vzLaunch(vzScene scene, const vzLaunchParams* params, vzStream stream, vzStats* stats)
 {
     if (!scene) return set_error(VZ_ERROR_INVALID_VALUE, "scene is NULL");
-    VZ_TRY(check_params(params));
+    VZ_TRY(check_launch_params(params));
     hipStream_t st = as_stream(stream);
     vzContext ctx = scene->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -338,20 +344,22 @@ vzStatus vzLaunch(vzScene scene, const vzLaunchParams* params, vzStream stream, 
         DevCounters c;
         VZ_HIP_TRY(hipMalloc(&c.d, 8 * sizeof(unsigned long long)));
         VZ_HIP_TRY(hipMemsetAsync(c.d, 0, 8 * sizeof(unsigned long long), st));
-        hipEvent_t e0, e1;
-        VZ_HIP_TRY(hipEventCreate(&e0));
-        VZ_HIP_TRY(hipEventCreate(&e1));
-        VZ_HIP_TRY(hipEventRecord(e0, st));
+        Event e0, e1;
+        VZ_EVENT_TRY(e0);
+        VZ_EVENT_TRY(e1);
+        VZ_HIP_TRY(hipEventRecord(e0.e, st));
         vzStatus rs = megakernel_launch(scene->view(), *params, st, c.d, (params->flags & VZ_COUNT_VISITS) != 0);
-        VZ_HIP_TRY(hipEventRecord(e1, st));
-        unsigned long long h[8];
-        VZ_HIP_TRY(hipMemcpyAsync(h, c.d, sizeof(h), hipMemcpyDeviceToHost, st));
-        VZ_HIP_TRY(hipStreamSynchronize(st));
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        unsigned long long h[8] = {};
+        if (rs == VZ_SUCCESS && hipEventRecord(e1.e, st) != hipSuccess)
+            rs = set_error(VZ_ERROR_HIP, "hipEventRecord failed");
+        if (rs == VZ_SUCCESS && hipMemcpyAsync(h, c.d, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess)
+            rs = set_error(VZ_ERROR_HIP, "hipMemcpyAsync of the counters failed");
+        // c.d is freed on return: the kernel must be done with it, on every path
+        if (hipStreamSynchronize(st) != hipSuccess && rs == VZ_SUCCESS)
+            rs = set_error(VZ_ERROR_HIP, "hipStreamSynchronize failed");
         VZ_TRY(rs);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0.e, e1.e);
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
             stats->segments = h[0];
@@ -422,7 +430,7 @@ vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n, cons
     for (uint32_t i = 0; i < n; i++) {
         if (!items[i].scene || items[i].scene->ctx != ctx)
             return set_error(VZ_ERROR_INVALID_VALUE, "batch item " + std::to_string(i) + " has no scene of this context");
-        VZ_TRY(check_params(&items[i].params));
+        VZ_TRY(check_launch_params(&items[i].params));
         if (items[i].params.flags & VZ_MEGAKERNEL)
             return set_error(VZ_ERROR_INVALID_VALUE, "VZ_MEGAKERNEL is not a batched mode");
         run[i].scene = items[i].scene;

@@ -921,16 +921,20 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
     const bool lds = single && scenes[0]->n >= 2 && scenes[0]->n - 1 <= kLdsNodes;
     if (use_graph) VZ_TRY(ensure_graph(e, opt.count_visits, lds, single));
     const uint32_t B = e->cfg.iters_per_graph;
-    hipEvent_t t0, t1, blk_ev[2];
-    VZ_HIP_TRY(hipEventCreate(&t0));
-    VZ_HIP_TRY(hipEventCreate(&t1));
-    VZ_HIP_TRY(hipEventCreateWithFlags(&blk_ev[0], hipEventDisableTiming));
-    VZ_HIP_TRY(hipEventCreateWithFlags(&blk_ev[1], hipEventDisableTiming));
-    std::vector<hipEvent_t> tev[2];
+    Event t0, t1, blk0(hipEventDisableTiming), blk1(hipEventDisableTiming);
+    VZ_EVENT_TRY(t0);
+    VZ_EVENT_TRY(t1);
+    VZ_EVENT_TRY(blk0);
+    VZ_EVENT_TRY(blk1);
+    const hipEvent_t blk_ev[2] = {blk0.e, blk1.e};
+    std::vector<Event> tev[2];  // per block parity: a pair of events around each step launch
     if (opt.time_trace) {
         for (int k = 0; k < 2; k++) {
-            tev[k].resize(2 * B);
-            for (auto& ev : tev[k]) VZ_HIP_TRY(hipEventCreate(&ev));
+            tev[k].reserve(2 * B);
+            for (uint32_t i = 0; i < 2 * B; i++) {
+                tev[k].emplace_back();
+                VZ_EVENT_TRY(tev[k].back());
+            }
         }
     }
     double trace_ms = 0.0;
@@ -940,8 +944,8 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
             VZ_HIP_TRY(hipGraphLaunch(e->graph, st));
         } else {
             for (uint32_t it = 0; it < B; it++) {
-                hipEvent_t a = opt.time_trace ? tev[k & 1][2 * it] : nullptr;
-                hipEvent_t b = opt.time_trace ? tev[k & 1][2 * it + 1] : nullptr;
+                hipEvent_t a = opt.time_trace ? tev[k & 1][2 * it].e : nullptr;
+                hipEvent_t b = opt.time_trace ? tev[k & 1][2 * it + 1].e : nullptr;
                 enqueue_iteration(e, st, (int)(it & 1u), opt.count_visits, lds, single, a, b);
             }
             VZ_HIP_TRY(hipGetLastError());
@@ -955,14 +959,14 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
         if (opt.time_trace) {
             for (uint32_t it = 0; it < B; it++) {
                 float ms = 0.0f;
-                VZ_HIP_TRY(hipEventElapsedTime(&ms, tev[k & 1][2 * it], tev[k & 1][2 * it + 1]));
+                VZ_HIP_TRY(hipEventElapsedTime(&ms, tev[k & 1][2 * it].e, tev[k & 1][2 * it + 1].e));
                 trace_ms += ms;
                 trace_launches++;
             }
         }
         return VZ_SUCCESS;
     };
-    VZ_HIP_TRY(hipEventRecord(t0, st));
+    VZ_HIP_TRY(hipEventRecord(t0.e, st));
     vzStatus rs = launch_block(0);
     uint32_t k = 0;
     const uint32_t max_blocks = 1u << 20;
@@ -982,21 +986,12 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
             break;
         }
     }
-    if (rs == VZ_SUCCESS) {
-        VZ_HIP_TRY(hipEventRecord(t1, st));
-        VZ_HIP_TRY(hipEventSynchronize(t1));
-    } else {
-        (void)hipStreamSynchronize(st);
-    }
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, t0, t1);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    (void)hipEventDestroy(blk_ev[0]);
-    (void)hipEventDestroy(blk_ev[1]);
-    for (int q = 0; q < 2; q++)
-        for (auto& ev : tev[q]) (void)hipEventDestroy(ev);
+    if (rs == VZ_SUCCESS && (hipEventRecord(t1.e, st) != hipSuccess || hipEventSynchronize(t1.e) != hipSuccess))
+        rs = set_error(VZ_ERROR_HIP, "hipEventRecord/Synchronize failed");
+    if (rs != VZ_SUCCESS) (void)hipStreamSynchronize(st);  // nothing in flight uses the events below
     VZ_TRY(rs);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, t0.e, t1.e);
     Counters fin;
     VZ_HIP_TRY(hipMemcpy(&fin, e->d_cnt, sizeof(Counters), hipMemcpyDeviceToHost));
     VZ_HIP_TRY(hipMemcpy(e->h_state, e->d_state, n * sizeof(ReqState), hipMemcpyDeviceToHost));

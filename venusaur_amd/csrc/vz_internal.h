@@ -3,9 +3,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/vz_api.h"
@@ -20,6 +22,9 @@ constexpr uint32_t kMaxRequests = 1024;  // device request-table size
 
 // Records a message for vzGetLastError and returns the status.
 vzStatus set_error(vzStatus s, const std::string& msg);
+// Validates a launch's parameters (sizes, samples, depth, buffers, a finite
+// camera frame); shared by vzLaunch, vzBatchLaunch and vzServerSubmit.
+vzStatus check_launch_params(const vzLaunchParams* p);
 
 #define VZ_HIP_TRY(expr)                                                              \
     do {                                                                              \
@@ -46,6 +51,34 @@ struct SceneView {           // what kernels need to traverse one scene
 
 struct Engine;
 
+// A HIP event owned by the enclosing scope, so early error returns do not
+// leak it.  ok() is false when creation failed.
+struct Event {
+    hipEvent_t e = nullptr;
+    explicit Event(unsigned flags = hipEventDefault)
+    {
+        if (hipEventCreateWithFlags(&e, flags) != hipSuccess) e = nullptr;
+    }
+    ~Event()
+    {
+        if (e) (void)hipEventDestroy(e);
+    }
+    Event(const Event&) = delete;
+    Event& operator=(const Event&) = delete;
+    Event(Event&& o) noexcept : e(o.e) { o.e = nullptr; }
+    Event& operator=(Event&& o) noexcept
+    {
+        std::swap(e, o.e);
+        return *this;
+    }
+    bool ok() const { return e != nullptr; }
+};
+
+#define VZ_EVENT_TRY(ev)                                                                  \
+    do {                                                                                  \
+        if (!(ev).ok()) return ::vz::set_error(VZ_ERROR_HIP, "hipEventCreate failed");    \
+    } while (0)
+
 }  // namespace vz
 
 struct vzContext_st {
@@ -57,6 +90,7 @@ struct vzContext_st {
     vzTuning tuning{};
     std::mutex mu;
     vz::Engine* engine = nullptr;  // lazily created wavefront engine
+    std::atomic<uint32_t> live_scenes{0}, live_servers{0};  // vzDestroy refuses while any remain
 };
 
 struct vzScene_st {

@@ -669,10 +669,10 @@ vzStatus lbvh_build(vzScene s, hipStream_t st)
 {
     uint32_t n = s->n;
     uint32_t nb = std::min<uint32_t>(1024, blocks_for(n, kBlk));
-    hipEvent_t e0, e1;
-    VZ_HIP_TRY(hipEventCreate(&e0));
-    VZ_HIP_TRY(hipEventCreate(&e1));
-    VZ_HIP_TRY(hipEventRecord(e0, st));
+    Event e0, e1;
+    VZ_EVENT_TRY(e0);
+    VZ_EVENT_TRY(e1);
+    VZ_HIP_TRY(hipEventRecord(e0.e, st));
     float* partial = s->d_bounds;
     float* fin = s->d_bounds + 6 * 1024;
     float* grid_b = fin + 16;
@@ -702,13 +702,11 @@ vzStatus lbvh_build(vzScene s, hipStream_t st)
     }
     VZ_HIP_TRY(hipGetLastError());
     vzStatus rs = refit_core(s, st);
-    VZ_HIP_TRY(hipEventRecord(e1, st));
-    VZ_HIP_TRY(hipEventSynchronize(e1));
+    VZ_HIP_TRY(hipEventRecord(e1.e, st));
+    VZ_HIP_TRY(hipEventSynchronize(e1.e));
     float ms = 0.0f;
-    VZ_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    VZ_HIP_TRY(hipEventElapsedTime(&ms, e0.e, e1.e));
     s->build_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (n < 2) s->depth = 1;
     return rs;
 }
@@ -732,21 +730,19 @@ vzStatus lbvh_check_input(vzScene s, const vzMaterial* mat, hipStream_t st, uint
 vzStatus lbvh_refit(vzScene s, hipStream_t st)
 {
     uint32_t n = s->n;
-    hipEvent_t e0, e1;
-    VZ_HIP_TRY(hipEventCreate(&e0));
-    VZ_HIP_TRY(hipEventCreate(&e1));
-    VZ_HIP_TRY(hipEventRecord(e0, st));
+    Event e0, e1;
+    VZ_EVENT_TRY(e0);
+    VZ_EVENT_TRY(e1);
+    VZ_HIP_TRY(hipEventRecord(e0.e, st));
     hipLaunchKernelGGL(k_reorder_sph, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in,
                        s->d_leaf_prim, n, s->d_sph);
     VZ_HIP_TRY(hipGetLastError());
     vzStatus rs = refit_core(s, st);
-    VZ_HIP_TRY(hipEventRecord(e1, st));
-    VZ_HIP_TRY(hipEventSynchronize(e1));
+    VZ_HIP_TRY(hipEventRecord(e1.e, st));
+    VZ_HIP_TRY(hipEventSynchronize(e1.e));
     float ms = 0.0f;
-    VZ_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    VZ_HIP_TRY(hipEventElapsedTime(&ms, e0.e, e1.e));
     s->build_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return rs;
 }
 

@@ -226,6 +226,7 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     }
     s->t0 = std::chrono::steady_clock::now();
     s->worker = std::thread([s] { s->loop(); });
+    ctx->live_servers++;
     *out = s;
     return VZ_SUCCESS;
 }
@@ -234,11 +235,7 @@ vzStatus vzServerSubmit(vzServer srv, vzScene scene, const vzLaunchParams* param
 {
     if (!srv || !scene || !params || !request_id) return set_error(VZ_ERROR_INVALID_VALUE, "NULL argument");
     if (scene->ctx != srv->ctx) return set_error(VZ_ERROR_INVALID_VALUE, "scene belongs to another context");
-    if (params->width < 2 || params->height < 2 || params->samples_per_pixel < 1 || params->max_depth < 1 ||
-        params->max_depth > 255 || !params->accum)
-        return set_error(VZ_ERROR_INVALID_VALUE, "invalid launch parameters");
-    if (params->samples_per_pixel > 1024 && !(params->flags & VZ_ACCUM_FP64))
-        return set_error(VZ_ERROR_INVALID_VALUE, "more than 1024 samples per pixel need VZ_ACCUM_FP64");
+    VZ_TRY(check_launch_params(params));
     if (params->flags & VZ_MEGAKERNEL) return set_error(VZ_ERROR_INVALID_VALUE, "VZ_MEGAKERNEL is not a serving mode");
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->fatal != VZ_SUCCESS) return set_error(srv->fatal, "server stopped: " + srv->fatal_msg);
@@ -291,6 +288,7 @@ vzStatus vzServerDestroy(vzServer srv)
     (void)hipStreamSynchronize(srv->stream);
     engine_destroy(srv->engine);
     (void)hipStreamDestroy(srv->stream);
+    srv->ctx->live_servers--;
     delete srv;
     return VZ_SUCCESS;
 }
