@@ -36,6 +36,17 @@ VZ_DEV float4 ldg(const float4* p)
     const raw_float4 v = *__builtin_bit_cast(global_float4*, p);  // a C-style cast would be dropped
     return make_float4(v.x, v.y, v.z, v.w);
 }
+typedef unsigned int raw_uint4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const raw_uint4 global_uint4;
+VZ_DEV uint4 ldg(const uint4* p)
+{
+    const raw_uint4 v = *__builtin_bit_cast(global_uint4*, p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+VZ_DEV uint32_t ldg(const uint32_t* p)
+{
+    return *__builtin_bit_cast(__attribute__((address_space(1))) const uint32_t*, p);
+}
 
 VZ_DEV v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
 VZ_DEV v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }

@@ -195,9 +195,9 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
                 nh++;
                 if (depth > 0u) {
                     // shade stage (RayTracer.cu:272-440)
-                    const float4 s = sc.spheres[tr.bleaf];
-                    const uint32_t type = sc.leaf_info[tr.bleaf] >> 30;
-                    const float4 mat = sc.mats[tr.bleaf];
+                    const float4 s = ldg(sc.spheres + tr.bleaf);
+                    const uint32_t type = ldg(sc.leaf_info + tr.bleaf) >> 30;
+                    const float4 mat = ldg(sc.mats + tr.bleaf);
                     pos = add(o, scl(d, tr.best));
                     v3 n = scl(sub(pos, mk(s.x, s.y, s.z)), 1.0f / s.w);
                     const bool front = dot(d, n) < 0.0f;

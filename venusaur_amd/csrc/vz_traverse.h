@@ -30,8 +30,8 @@ struct GlobalNodes {
     const uint4* p;
     VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b) const
     {
-        a = p[2 * i];
-        b = p[2 * i + 1];
+        a = ldg(p + 2 * i);
+        b = ldg(p + 2 * i + 1);
     }
 };
 struct LdsNodes {
@@ -142,8 +142,8 @@ struct Traversal {
                 bleaf = leaf;
                 bprim = 0xFFFFFFFFu;  // looked up lazily on a tie
             } else {
-                const uint32_t pid = sc.leaf_info[leaf] & 0x3FFFFFFFu;
-                if (bprim == 0xFFFFFFFFu && bleaf != 0xFFFFFFFFu) bprim = sc.leaf_info[bleaf] & 0x3FFFFFFFu;
+                const uint32_t pid = ldg(sc.leaf_info + leaf) & 0x3FFFFFFFu;
+                if (bprim == 0xFFFFFFFFu && bleaf != 0xFFFFFFFFu) bprim = ldg(sc.leaf_info + bleaf) & 0x3FFFFFFFu;
                 if (pid < bprim) {
                     best = t;
                     bleaf = leaf;
@@ -197,17 +197,20 @@ struct Traversal {
             tn[c] = lo_t;
             hit[c] = lo_t <= hi_t * kRobust;
         }
-        // guards, checked after the box arithmetic so both node loads issue
-        // together: children must be valid indices, and a well-formed tree
-        // visits each of its n-1 internal nodes at most once
-        if (g + 1u >= sc.n || budget-- == 0u) return true;
+        // guards: children must be valid indices, and a well-formed tree
+        // visits each of its n-1 internal nodes at most once.  Folded in as a
+        // select rather than an early return, so that the compiler cannot
+        // move the box arithmetic, and with it the first node load, behind a
+        // wait for the second.
+        const bool bad = g + 1u >= sc.n || budget == 0u;
+        budget--;
         const uint32_t leaf = (meta >> 24) & 3u;
         const uint32_t h = (hit[0] ? 1u : 0u) | (hit[1] ? 2u : 0u);
-        pend = (h & leaf) | ((h & ~leaf & 3u) << 2);
+        pend = bad ? 0u : (h & leaf) | ((h & ~leaf & 3u) << 2);
         pg = g;
         ptn0 = tn[0];
         ptn1 = tn[1];
-        return false;
+        return bad;
     }
     VZ_DEV bool has_leaves() const { return (pend & 3u) != 0u; }
 
