@@ -43,6 +43,16 @@ VZ_DEV uint4 ldg(const uint4* p)
     const raw_uint4 v = *__builtin_bit_cast(global_uint4*, p);
     return make_uint4(v.x, v.y, v.z, v.w);
 }
+// Two consecutive float4 in one load (16-byte aligned), so both halves are in
+// flight before the first wait.
+typedef float raw_float8 __attribute__((ext_vector_type(8), aligned(16)));
+typedef __attribute__((address_space(1))) const raw_float8 global_float8;
+VZ_DEV void ldg2(const float4* p, float4& a, float4& b)
+{
+    const raw_float8 v = *__builtin_bit_cast(global_float8*, p);
+    a = make_float4(v.s0, v.s1, v.s2, v.s3);
+    b = make_float4(v.s4, v.s5, v.s6, v.s7);
+}
 VZ_DEV uint32_t ldg(const uint32_t* p)
 {
     return *__builtin_bit_cast(__attribute__((address_space(1))) const uint32_t*, p);

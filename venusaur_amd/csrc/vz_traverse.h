@@ -131,12 +131,13 @@ struct Traversal {
         return lo_t <= hi_t * kRobust;
     }
 
-    // Sphere test of leaf `leaf`; keeps the lexicographic minimum of (t, prim).
-    VZ_DEV void test_leaf(const SceneView& sc, uint32_t leaf, uint32_t& ns)
+    // Sphere test of leaf `leaf` (sphere s); keeps the lexicographic minimum
+    // of (t, prim).
+    VZ_DEV void test_leaf(const SceneView& sc, uint32_t leaf, float4 s, uint32_t& ns)
     {
         float t;
         if (COUNT) ns++;
-        if (hit_gated(ldg(sc.spheres + leaf), t)) {
+        if (hit_gated(s, t)) {
             if (t < best) {
                 best = t;
                 bleaf = leaf;
@@ -214,11 +215,16 @@ struct Traversal {
     }
     VZ_DEV bool has_leaves() const { return (pend & 3u) != 0u; }
 
-    // Sphere tests of the pending visit's leaf children, in child order.
+    // Sphere tests of the pending visit's leaf children, in child order.  Both
+    // children's spheres come in one load: pg + 1 < n holds after a visit that
+    // did not end the traversal.
     VZ_DEV void leaves(const SceneView& sc, uint32_t& ns)
     {
-        if (pend & 1u) test_leaf(sc, pg, ns);
-        if (pend & 2u) test_leaf(sc, pg + 1u, ns);
+        if ((pend & 3u) == 0u) return;
+        float4 s0, s1;
+        ldg2(sc.spheres + pg, s0, s1);
+        if (pend & 1u) test_leaf(sc, pg, s0, ns);
+        if (pend & 2u) test_leaf(sc, pg + 1u, s1, ns);
     }
 
     // Next node after the pending visit: the near internal child (the far one
