@@ -69,7 +69,7 @@ def _build_scenes(ctx, reqs):
         if k % 200 == 0:
             _progress(f"building scenes: request {k}/{len(reqs)}")
         if r.scene_key not in scenes:
-            s, m = api.box_scene(r.n_small, 1000 + r.scene_key)
+            s, m = r.scene_arrays()
             scenes[r.scene_key] = api.Scene(ctx, s, m)
             build_ms.append(scenes[r.scene_key].info().build_ms)
     return scenes, build_ms
@@ -78,7 +78,8 @@ def _build_scenes(ctx, reqs):
 def _rmse_fp64(r):
     """RMSE of a bound request's GPU accumulator against the FP64 oracle."""
     import oracle as O
-    s, m = O.box_scene(r.n_small, 1000 + r.scene_key)
+    from venusaur_amd import workloads
+    s, m = O.box_scene(r.n_small, workloads.SCENE_SEED_BASE + r.scene_key)
     cam = O.camera_lookat(list(r.lookfrom), list(r.lookat), r.vfov, float(np.float32(r.width) / np.float32(r.height)),
                           0.0, 10.0)
     p = O.make_params(r.width, r.height, r.spp, r.depth, cam)

@@ -202,6 +202,63 @@ vzStatus vzServerDestroy(vzServer srv);
 /* Bytes of device memory a request of these params needs (admission). */
 uint64_t vzRequestFootprint(const vzLaunchParams* params);
 
+/* ---- wire format (input of the serving form) ------------------------
+ * The reference has no scene file: its scene is code (Core/Scene.h:13-80)
+ * and its request is the Params a Draw fills in (Core/RayTracer.h:3-17,
+ * Core/Renderer.h:37-75).  These records carry the same content between
+ * processes.  Little-endian, 4-byte aligned:
+ *
+ *   scene record (24 + 40 n bytes)
+ *     0  "VZSC"   4  u32 version (1)   8  u32 n   12  u32 0
+ *     16 u64 FNV-1a 64 of the payload
+ *     24 n x vzSphere (16 B), then n x vzMaterial (24 B)
+ *   request record (96 bytes, then scene_bytes of an inline scene record)
+ *     0  "VZRQ"   4  u32 version (1)   8  u64 tag   16  u64 scene_key
+ *     24 u32 width, height, samples_per_pixel, max_depth
+ *     40 u32 flags (VZ_ACCUM_FP64 or 0)   44 u32 subframe_index
+ *     48 f32 lookfrom[3]   60 f32 lookat[3]
+ *     72 f32 vfov_deg, aperture, focus_dist, arrival_ms
+ *     88 u32 scene_bytes (0: the key names a scene already resident)   92 u32 0
+ *
+ * Decoders check magic, version, sizes, the checksum and every value a
+ * launch would reject, and never read past `size`. */
+#define VZ_WIRE_SCENE_HEADER 24u
+#define VZ_WIRE_REQUEST_HEADER 96u
+
+typedef struct vzRequestDesc {
+    uint64_t tag;                 /* caller's id, echoed back */
+    uint64_t scene_key;           /* names the scene; requests may share one */
+    uint32_t width, height, samples_per_pixel, max_depth;
+    uint32_t flags;               /* 0 or VZ_ACCUM_FP64 */
+    uint32_t subframe_index;      /* >= 1 */
+    float lookfrom[3], lookat[3]; /* camera (Camera.cpp:24-37), up = +y */
+    float vfov_deg, aperture, focus_dist;
+    float arrival_ms;             /* offset from the start of the stream (replay) */
+} vzRequestDesc;
+
+/* Bytes of a scene record of n spheres. */
+size_t vzWireSceneSize(uint32_t n);
+/* Writes a scene record into buf (cap bytes); *written = its size. */
+vzStatus vzWireEncodeScene(const vzSphere* spheres, const vzMaterial* materials, uint32_t n, void* buf,
+                           size_t cap, size_t* written);
+/* Reads the scene record at the start of buf.  *n = its sphere count; with
+ * spheres and materials non-NULL (room for cap each) also copies them out.
+ * *consumed = the record's size. */
+vzStatus vzWireDecodeScene(const void* buf, size_t size, vzSphere* spheres, vzMaterial* materials,
+                           uint32_t cap, uint32_t* n, size_t* consumed);
+/* Writes a request record, followed by a scene record of n spheres when
+ * n > 0 (n = 0: the request names a resident scene by its key). */
+vzStatus vzWireEncodeRequest(const vzRequestDesc* req, const vzSphere* spheres, const vzMaterial* materials,
+                             uint32_t n, void* buf, size_t cap, size_t* written);
+/* Reads the request record at the start of buf.  *scene_bytes = the size of
+ * its inline scene record (0 if none), which starts at buf + 96; *consumed =
+ * 96 + *scene_bytes. */
+vzStatus vzWireDecodeRequest(const void* buf, size_t size, vzRequestDesc* out, size_t* scene_bytes,
+                             size_t* consumed);
+/* Launch params of a request (camera by vzCameraLookAt, aspect = width /
+ * height in float) for the given device buffers. */
+vzStatus vzRequestLaunchParams(const vzRequestDesc* req, void* image, void* accum, vzLaunchParams* out);
+
 /* ---- host helpers (scene generators and camera, Scene.h / Camera.cpp) -- */
 /* Book final scene (Scene.h:13-80) from std::mt19937(seed), draws in
  * x -> y -> z order.  Writes at most cap spheres; *n_out = count. */

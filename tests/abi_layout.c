@@ -13,7 +13,7 @@ int main(void)
 {
     printf("{");
     S(vzSphere); S(vzMaterial); S(vzRay); S(vzHit); S(vzLaunchParams); S(vzStats);
-    S(vzAccelInfo); S(vzBatchItem); S(vzTuning); S(vzServeConfig); S(vzRequestResult);
+    S(vzAccelInfo); S(vzBatchItem); S(vzTuning); S(vzServeConfig); S(vzRequestResult); S(vzRequestDesc);
     F(vzLaunchParams, image); F(vzLaunchParams, accum); F(vzLaunchParams, width);
     F(vzLaunchParams, samples_per_pixel); F(vzLaunchParams, subframe_index); F(vzLaunchParams, origin);
     F(vzLaunchParams, u); F(vzLaunchParams, w); F(vzLaunchParams, lens_radius);
@@ -23,6 +23,9 @@ int main(void)
     F(vzRay, tmin); F(vzRay, dir); F(vzRay, tmax);
     F(vzBatchItem, params); F(vzServeConfig, tuning); F(vzRequestResult, segments);
     F(vzRequestResult, status);
+    F(vzRequestDesc, scene_key); F(vzRequestDesc, width); F(vzRequestDesc, flags);
+    F(vzRequestDesc, subframe_index); F(vzRequestDesc, lookfrom); F(vzRequestDesc, lookat);
+    F(vzRequestDesc, vfov_deg); F(vzRequestDesc, focus_dist); F(vzRequestDesc, arrival_ms);
     printf("\"end\": 0}\n");
     return 0;
 }

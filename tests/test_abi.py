@@ -39,7 +39,7 @@ def test_struct_layout_matches_c_compiler(tmp_path):
     assert lay["vzHit"] == api.HIT_DT.itemsize == 8
     for name, cls in [("vzLaunchParams", api.LaunchParams), ("vzStats", api.Stats), ("vzAccelInfo", api.AccelInfo),
                       ("vzBatchItem", api.BatchItem), ("vzTuning", api.Tuning), ("vzServeConfig", api.ServeConfig),
-                      ("vzRequestResult", api.RequestResult)]:
+                      ("vzRequestResult", api.RequestResult), ("vzRequestDesc", api.RequestDesc)]:
         assert lay[name] == C.sizeof(cls), name
         for key, off in lay.items():
             if key.startswith(name + "."):

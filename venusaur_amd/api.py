@@ -77,6 +77,15 @@ class ServeConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_uint32), ("tuning", Tuning)]
 
 
+class RequestDesc(C.Structure):
+    """vzRequestDesc: one render request of the wire format."""
+    _fields_ = [("tag", C.c_uint64), ("scene_key", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("samples_per_pixel", C.c_uint32), ("max_depth", C.c_uint32), ("flags", C.c_uint32),
+                ("subframe_index", C.c_uint32), ("lookfrom", C.c_float * 3), ("lookat", C.c_float * 3),
+                ("vfov_deg", C.c_float), ("aperture", C.c_float), ("focus_dist", C.c_float),
+                ("arrival_ms", C.c_float)]
+
+
 class RequestResult(C.Structure):
     _fields_ = [("request_id", C.c_uint64), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
                 ("finish_ms", C.c_double), ("segments", C.c_uint64), ("status", C.c_uint32)]
@@ -115,6 +124,12 @@ def _declare(L):
         "vzGenerateBookScene": (st, [u32, vp, vp, u32, P(u32)]),
         "vzGenerateBoxScene": (st, [u32, u32, vp, vp]),
         "vzCameraLookAt": (st, [vp, vp, f32, f32, f32, f32, P(LaunchParams)]),
+        "vzWireSceneSize": (C.c_size_t, [u32]),
+        "vzWireEncodeScene": (st, [vp, vp, u32, vp, C.c_size_t, P(C.c_size_t)]),
+        "vzWireDecodeScene": (st, [vp, C.c_size_t, vp, vp, u32, P(u32), P(C.c_size_t)]),
+        "vzWireEncodeRequest": (st, [P(RequestDesc), vp, vp, u32, vp, C.c_size_t, P(C.c_size_t)]),
+        "vzWireDecodeRequest": (st, [vp, C.c_size_t, P(RequestDesc), P(C.c_size_t), P(C.c_size_t)]),
+        "vzRequestLaunchParams": (st, [P(RequestDesc), vp, vp, P(LaunchParams)]),
         "vzStatusString": (C.c_char_p, [st]),
         "vzGetLastError": (C.c_char_p, []),
     }
@@ -177,6 +192,86 @@ def launch_params(width, height, spp, max_depth, lookfrom, lookat, vfov, apertur
 
 def footprint(params):
     return lib().vzRequestFootprint(C.byref(params))
+
+
+# ---- wire format (input of the serving form) -------------------------------------
+WIRE_SCENE_HEADER = 24
+WIRE_REQUEST_HEADER = 96
+
+
+def _bytes_ptr(b):
+    buf = C.create_string_buffer(bytes(b), len(b)) if len(b) else C.create_string_buffer(1)
+    return buf, C.cast(buf, C.c_void_p)
+
+
+def encode_scene(spheres, materials):
+    """Scene record (bytes) of the sphere / material arrays."""
+    s = np.ascontiguousarray(spheres, SPHERE_DT)
+    m = np.ascontiguousarray(materials, MATERIAL_DT)
+    if len(s) != len(m):
+        raise ValueError("spheres and materials differ in length")
+    size = lib().vzWireSceneSize(len(s))
+    out = C.create_string_buffer(size)
+    w = C.c_size_t(0)
+    check(lib().vzWireEncodeScene(_p(s), _p(m), len(s), C.cast(out, C.c_void_p), size, C.byref(w)))
+    return out.raw[:w.value]
+
+
+def decode_scene(data):
+    """(spheres, materials, consumed) of the scene record at the start of data."""
+    buf, ptr = _bytes_ptr(data)
+    n = C.c_uint32(0)
+    used = C.c_size_t(0)
+    check(lib().vzWireDecodeScene(ptr, len(data), None, None, 0, C.byref(n), C.byref(used)))
+    s = np.zeros(n.value, SPHERE_DT)
+    m = np.zeros(n.value, MATERIAL_DT)
+    check(lib().vzWireDecodeScene(ptr, len(data), _p(s), _p(m), n.value, C.byref(n), C.byref(used)))
+    return s, m, used.value
+
+
+def request_desc(tag, scene_key, width, height, spp, max_depth, lookfrom, lookat, vfov, aperture=0.0,
+                 focus=10.0, flags=0, subframe=1, arrival_ms=0.0):
+    d = RequestDesc()
+    d.tag, d.scene_key = tag, scene_key
+    d.width, d.height, d.samples_per_pixel, d.max_depth = width, height, spp, max_depth
+    d.flags, d.subframe_index = flags, subframe
+    d.lookfrom[:] = [float(x) for x in lookfrom]
+    d.lookat[:] = [float(x) for x in lookat]
+    d.vfov_deg, d.aperture, d.focus_dist, d.arrival_ms = vfov, aperture, focus, arrival_ms
+    return d
+
+
+def encode_request(desc, spheres=None, materials=None):
+    """Request record (bytes), with an inline scene record when spheres is given."""
+    n = 0
+    sp = mp = None
+    if spheres is not None:
+        s = np.ascontiguousarray(spheres, SPHERE_DT)
+        m = np.ascontiguousarray(materials, MATERIAL_DT)
+        if len(s) != len(m):
+            raise ValueError("spheres and materials differ in length")
+        n, sp, mp = len(s), _p(s), _p(m)
+    size = WIRE_REQUEST_HEADER + (lib().vzWireSceneSize(n) if n else 0)
+    out = C.create_string_buffer(size)
+    w = C.c_size_t(0)
+    check(lib().vzWireEncodeRequest(C.byref(desc), sp, mp, n, C.cast(out, C.c_void_p), size, C.byref(w)))
+    return out.raw[:w.value]
+
+
+def decode_request(data):
+    """(RequestDesc, inline scene bytes, consumed) of the request record at the start of data."""
+    buf, ptr = _bytes_ptr(data)
+    d = RequestDesc()
+    sb = C.c_size_t(0)
+    used = C.c_size_t(0)
+    check(lib().vzWireDecodeRequest(ptr, len(data), C.byref(d), C.byref(sb), C.byref(used)))
+    return d, bytes(data[WIRE_REQUEST_HEADER:WIRE_REQUEST_HEADER + sb.value]), used.value
+
+
+def request_launch_params(desc, image=None, accum=None):
+    p = LaunchParams()
+    check(lib().vzRequestLaunchParams(C.byref(desc), image, accum, C.byref(p)))
+    return p
 
 
 # ---- device objects --------------------------------------------------------------
@@ -357,4 +452,4 @@ def declared_symbols():
     """Function names declared in include/vz_api.h."""
     import re
     text = open(HEADER_PATH).read()
-    return sorted(set(re.findall(r"^(?:vzStatus|uint64_t|const char\*)\s+(vz\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:vzStatus|uint64_t|size_t|const char\*)\s+(vz\w+)\(", text, re.M)))

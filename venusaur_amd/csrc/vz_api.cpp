@@ -24,7 +24,7 @@ static hipStream_t as_stream(vzStream s) { return reinterpret_cast<hipStream_t>(
 
 static bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
-static vzStatus check_spheres(const vzSphere* s, uint32_t n)
+vzStatus check_spheres(const vzSphere* s, uint32_t n)
 {
     for (uint32_t i = 0; i < n; i++) {
         if (!finite3(s[i].center) || !std::isfinite(s[i].radius) || s[i].radius == 0.0f)

@@ -25,6 +25,8 @@ vzStatus set_error(vzStatus s, const std::string& msg);
 // Validates a launch's parameters (sizes, samples, depth, buffers, a finite
 // camera frame); shared by vzLaunch, vzBatchLaunch and vzServerSubmit.
 vzStatus check_launch_params(const vzLaunchParams* p);
+// Rejects spheres with a non-finite centre or a zero / non-finite radius.
+vzStatus check_spheres(const vzSphere* s, uint32_t n);
 
 #define VZ_HIP_TRY(expr)                                                              \
     do {                                                                              \

@@ -9,6 +9,10 @@ Stream: requests from the batch distribution with Poisson arrivals; a
 fraction reuses an already resident scene with a random camera (lookfrom on
 a sphere of radius U[15,150], vfov U[20,60]); a fraction are 512x512 at
 2048 samples per pixel with FP64 accumulation.
+
+Scene of a request: the random-box generator with seed SCENE_SEED_BASE +
+scene_key.  write_stream() turns requests into wire-format request records
+(include/vz_api.h), each scene inline in the first record that names it.
 """
 import math
 from dataclasses import dataclass, field
@@ -20,6 +24,7 @@ from .hipmem import DeviceBuffer
 
 BOX_FROM = (0.0, 15.0, -120.0)
 BOX_AT = (0.0, 0.0, 0.0)
+SCENE_SEED_BASE = 1000
 
 
 @dataclass
@@ -58,6 +63,15 @@ class Request:
 
     def accum_host(self):
         return self.accum.download(np.float64 if self.fp64 else np.float32, (self.height, self.width, 4))
+
+    def scene_arrays(self):
+        return api.box_scene(self.n_small, SCENE_SEED_BASE + self.scene_key)
+
+    def desc(self, tag):
+        """The request as a wire-format vzRequestDesc (aperture 0, focus 10, subframe 1)."""
+        return api.request_desc(tag, self.scene_key, self.width, self.height, self.spp, self.depth,
+                                self.lookfrom, self.lookat, self.vfov, 0.0, 10.0,
+                                flags=api.ACCUM_FP64 if self.fp64 else 0, arrival_ms=self.arrival_s * 1e3)
 
 
 def _scene_size(rng):
@@ -109,3 +123,22 @@ def request_stream(n, rate_hz, seed=11, reuse_frac=0.2, fp64_frac=0.05):
         if not req.reuse and not req.fp64:
             resident.append(req)
     return out
+
+
+def write_stream(path, requests):
+    """Writes requests as wire-format request records (tag = index in the
+    list); a scene goes inline in the first record that names its key.
+    Returns the number of bytes written."""
+    seen = set()
+    total = 0
+    with open(path, "wb") as f:
+        for i, r in enumerate(requests):
+            if r.scene_key in seen:
+                rec = api.encode_request(r.desc(i))
+            else:
+                s, m = r.scene_arrays()
+                rec = api.encode_request(r.desc(i), s, m)
+                seen.add(r.scene_key)
+            f.write(rec)
+            total += len(rec)
+    return total
