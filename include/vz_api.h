@@ -167,7 +167,7 @@ typedef struct vzTuning {
  * items share the ray/shade/miss queues.  Synchronises the stream. */
 vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n,
                        const vzTuning* tuning, vzStream stream, vzStats* stats);
-/* Renders the batch once per candidate tuning (queue capacity 4M/8M/16M rays
+/* Renders the batch once per candidate tuning (queue capacity 4M/8M/12M rays
  * x camera tiles 32x2/8x8), timing the wavefront only, and returns the
  * fastest; caches the choice per context.  The items' accumulators are
  * written (as by vzBatchLaunch). */

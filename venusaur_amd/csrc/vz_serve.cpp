@@ -302,7 +302,7 @@ vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStrea
     // favoured small queues that lost on the real batch.
     std::vector<vzBatchItem> probe(items, items + n);
     for (auto& it : probe) it.params.flags &= ~(uint32_t)(VZ_COUNT_VISITS | VZ_TIME_STAGES);
-    const uint32_t caps[] = {1u << 22, 1u << 23, 1u << 24};
+    const uint32_t caps[] = {1u << 22, 1u << 23, 3u << 22};
     const uint32_t tiles[][2] = {{32, 2}, {8, 8}};
     vzTuning best{};
     double best_rate = -1.0;
