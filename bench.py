@@ -62,6 +62,31 @@ def _progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _traffic(name, avg_ms):
+    """Measured HBM traffic and L2 figures of a step kernel from its committed
+    rocprofv3 summary (profiles/*.json), or Nones."""
+    tf = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(tf):
+        return {"traffic": None, "measured_hbm_gbs": None}
+    j = json.load(open(tf))
+    out = {"traffic": j.get("traffic_bytes_per_launch"),
+           "measured_hbm_gbs": round(j["traffic_bytes_per_launch"] / (avg_ms * 1e-3) / 1e9, 1)}
+    if j.get("l2_hit_rate") is not None:
+        out["l2_hit_rate"] = j["l2_hit_rate"]
+        out["l2_requests_per_launch"] = j["l2_requests_per_launch"]
+    return out
+
+
 def _build_scenes(ctx, reqs):
     from venusaur_amd import api
     scenes, build_ms = {}, []
@@ -176,11 +201,7 @@ def run_c2(ctx, frames, seed=2024):
            "step_kernel": {"launches": launches, "avg_launch_ms": round(avg_ms, 4),
                            "algorithmic_bytes_per_launch": round(per_launch),
                            "algorithmic_gbs": round(per_launch / (avg_ms * 1e-3) / 1e9, 1)}}
-    tf = os.path.join(ROOT, "profiles", "step_kernel_traffic_c2.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("traffic_bytes_per_launch")
-        out["step_kernel"]["traffic"] = traffic
-        out["step_kernel"]["measured_hbm_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+    out["step_kernel"].update(_traffic("step_kernel_traffic_c2.json", avg_ms))
     acc.free()
     img.free()
     sc.close()
@@ -337,14 +358,11 @@ def main():
         # measured HBM bytes per launch of the same kernel on the same
         # workload: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy
         # kernel of known size (profiles/k_wf_step_book_1080p_256spp.md)
-        traffic = measured_gbs = None
-        tf = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
-        if os.path.exists(tf) and (W, H, SPP, DEPTH) == (1920, 1080, 256, 50):
-            traffic = json.load(open(tf)).get("traffic_bytes_per_launch")
-            measured_gbs = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+        meas = {"traffic": None, "measured_hbm_gbs": None}
+        if (W, H, SPP, DEPTH) == (1920, 1080, 256, 50):
+            meas = _traffic("step_kernel_traffic.json", avg_ms)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "measured_hbm_gbs": measured_gbs,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), **meas,
                 "algorithmic_bytes_per_launch": round(per_launch_bytes),
                 "kernel": "k_wf_step", "launches": launches, "avg_launch_ms": round(avg_ms, 4),
                 "trace_share_of_step": round(trace_ms / (sum(s.render_ms for s in stats)), 3),
@@ -371,7 +389,8 @@ def main():
                "kind": "port",
                "sample": f"rows y%{a.cpu_row_step}==0 ({len(rows)} of {H} rows, full {SPP} spp, depth {DEPTH}): "
                          f"{cst_cpu.segments} segments in {cst_cpu.seconds:.1f} s",
-               "gpu_over_cpu": round(value / cpu_rate, 1)}
+               "gpu_over_cpu": round(value / cpu_rate, 1), "cpu_model": _cpu_model(),
+               "host_cpus": os.cpu_count()}
         checks["rmse_vs_cpu_fp32_sample_rows"] = float(np.sqrt(np.mean(
             (gpu_acc[rows, :, :3].astype(np.float64) - cacc[rows, :, :3]) ** 2)))
         checks["bit_exact_sample_rows"] = bool(np.array_equal(gpu_acc[rows], cacc[rows]))

@@ -88,7 +88,7 @@ struct Plan {
 };
 
 struct EngineConfig {
-    uint32_t queue_capacity = 3u << 22;  // rays in flight (12M)
+    uint32_t queue_capacity = 1u << 23;  // rays in flight (8M)
     uint32_t tile_w = 32, tile_h = 2;   // one wave of camera rays = 32x2 pixels
     uint32_t iters_per_graph = 16;
     uint32_t refill_lanes = 0;          // 0: per-variant default (kernel side)
