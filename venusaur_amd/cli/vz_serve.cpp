@@ -6,6 +6,9 @@
 //   vz_serve --in FILE [--budget-mb N] [--max-active N] [--realtime]
 //            [--accum-dir DIR] [--ppm-dir DIR]
 //
+// FILE is a regular file of concatenated request records (its size bounds
+// every record's inline scene).  Each request gets a cleared accumulator, so
+// a subframe_index above 1 blends into zeros.
 // --realtime submits each request at its arrival_ms after the start; without
 // it every request is submitted as soon as it is read.  A request whose key is
 // not resident must carry its scene inline; an inline scene for a resident
@@ -243,7 +246,10 @@ int main(int argc, char** argv)
             const size_t abytes = npx * ((r.flags & VZ_ACCUM_FP64) ? 32 : 16);
             hck(hipMalloc(&j.accum, abytes), "accum allocation");
             hck(hipMalloc(&j.image, npx * 4), "image allocation");
-            hck(hipMemset(j.accum, 0, abytes), "accum clear");
+            // the server renders on its own non-blocking stream: the clear must
+            // be complete before the request is submitted
+            hck(hipMemsetAsync(j.accum, 0, abytes, nullptr), "accum clear");
+            hck(hipStreamSynchronize(nullptr), "accum clear");
             vzLaunchParams p;
             ck(vzRequestLaunchParams(&r, j.image, j.accum, &p), "vzRequestLaunchParams");
             const vzStatus sub = vzServerSubmit(srv, sc, &p, &j.id);
