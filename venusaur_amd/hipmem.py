@@ -89,7 +89,10 @@ class DeviceBuffer:
         return out.reshape(shape) if shape is not None else out
 
     def zero(self):
+        """Clears the buffer and waits for the clear: the serving loop renders
+        on its own non-blocking stream, which does not wait for the null one."""
         _ck(hip().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
+        _ck(hip().hipStreamSynchronize(None), "hipStreamSynchronize")
 
     def free(self):
         if self.ptr:
