@@ -223,3 +223,41 @@ def test_request_camera_and_scene_size_checks():
     # a request with 2048 spp is valid with FP64 accumulation
     d.samples_per_pixel, d.flags = 2048, api.ACCUM_FP64
     assert api.decode_request(api.encode_request(d))[0].samples_per_pixel == 2048
+
+
+SERVE = os.path.join(os.path.dirname(HERE), "venusaur_amd", "bin", "vz_serve")
+
+
+def _serve(*args):
+    import subprocess
+    assert os.path.exists(SERVE), "venusaur_amd/bin/vz_serve is missing: build it with __graft_entry__.build()"
+    return subprocess.run([SERVE, *args], capture_output=True, text=True, timeout=60)
+
+
+def test_vz_serve_usage_errors(tmp_path):
+    assert _serve().returncode == 2
+    assert _serve("--in", str(tmp_path / "missing.vzr")).returncode == 2
+    assert _serve("--in", "x", "--bogus").returncode == 2
+
+
+def test_vz_serve_rejects_a_malformed_stream_before_any_gpu_work(tmp_path):
+    """Each record is decoded before the device is touched: these fail with the
+    record's own message, not a device error, with or without a GPU."""
+    s, m = _small_scene()
+    good = api.encode_request(api.request_desc(0, 1, 32, 32, 2, 4, [0, 1, -5], [0, 0, 0], 40.0), s, m)
+    second = api.encode_request(api.request_desc(1, 1, 32, 32, 2, 4, [0, 2, -5], [0, 0, 0], 40.0))
+    flipped = bytearray(good)
+    flipped[96 + 30] ^= 0x01  # inside the inline scene's payload
+    cases = {
+        "empty": (b"", "no request record"),
+        "truncated header": (good + second[:50], "record 1: truncated header"),
+        "inline past the end": (good[:-8], "record 0: inline scene runs past the end"),
+        "checksum": (bytes(flipped) + second, "record 0: scene record checksum mismatch"),
+        "bad value": (good + _patch(second, 32, "<I", 0), "record 1: samples_per_pixel"),
+    }
+    for name, (data, msg) in cases.items():
+        path = tmp_path / (name.replace(" ", "_") + ".vzr")
+        path.write_bytes(data)
+        r = _serve("--in", str(path))
+        assert r.returncode == 1 and msg in r.stderr, (name, r.stderr)
+        assert "vzCreate" not in r.stderr, name

@@ -7,8 +7,10 @@
 //            [--accum-dir DIR] [--ppm-dir DIR]
 //
 // FILE is a regular file of concatenated request records (its size bounds
-// every record's inline scene).  Each request gets a cleared accumulator, so
-// a subframe_index above 1 blends into zeros.
+// every record's inline scene).  Every record is decoded once before any GPU
+// work, so a malformed stream fails before the first request runs.  Each
+// request gets a cleared accumulator, so a subframe_index above 1 blends into
+// zeros.
 // --realtime submits each request at its arrival_ms after the start; without
 // it every request is submitted as soon as it is read.  A request whose key is
 // not resident must carry its scene inline; an inline scene for a resident
@@ -153,6 +155,43 @@ void reap(const Args& a, vzServer srv, std::vector<Job>& jobs, size_t& first_ope
     while (first_open < jobs.size() && jobs[first_open].done) first_open++;
 }
 
+// Reads the next request record (header and inline scene) into rec; false
+// at the end of the file.  Throws on a truncated record or an inline scene
+// that runs past the end of the file.
+bool read_record(FILE* in, uint64_t file_size, uint64_t& offset, uint64_t index, std::vector<unsigned char>& rec)
+{
+    rec.resize(VZ_WIRE_REQUEST_HEADER);
+    const size_t got = std::fread(rec.data(), 1, VZ_WIRE_REQUEST_HEADER, in);
+    if (got == 0) return false;
+    const std::string where = "record " + std::to_string(index);
+    if (got < VZ_WIRE_REQUEST_HEADER) throw Fail{where + ": truncated header"};
+    uint32_t inline_bytes = 0;  // offset 88 of the request header
+    std::memcpy(&inline_bytes, rec.data() + 88, 4);
+    offset += VZ_WIRE_REQUEST_HEADER;
+    if (inline_bytes > file_size - offset) throw Fail{where + ": inline scene runs past the end of the file"};
+    offset += inline_bytes;
+    rec.resize(VZ_WIRE_REQUEST_HEADER + (size_t)inline_bytes);
+    if (inline_bytes && std::fread(rec.data() + VZ_WIRE_REQUEST_HEADER, 1, inline_bytes, in) != inline_bytes)
+        throw Fail{where + ": truncated inline scene"};
+    return true;
+}
+
+// First pass: every record decodes (sizes, checksums, values), before any
+// GPU work.  Returns the number of records.
+uint64_t validate_stream(FILE* in, uint64_t file_size)
+{
+    uint64_t offset = 0, index = 0;
+    std::vector<unsigned char> rec;
+    vzRequestDesc d;
+    while (read_record(in, file_size, offset, index, rec)) {
+        if (vzWireDecodeRequest(rec.data(), rec.size(), &d, nullptr, nullptr) != VZ_SUCCESS)
+            throw Fail{"record " + std::to_string(index) + ": " + vzGetLastError()};
+        index++;
+    }
+    if (index == 0) throw Fail{"the stream holds no request record"};
+    return index;
+}
+
 double pct(std::vector<double> v, double q)
 {
     if (v.empty()) return 0.0;
@@ -180,6 +219,13 @@ int main(int argc, char** argv)
     uint64_t inline_built = 0, inline_ignored = 0;
     double wall_s = 0.0;
     try {
+        std::fseek(in, 0, SEEK_END);
+        const long end = std::ftell(in);
+        if (end < 0) throw Fail{a.in + " is not a seekable file"};
+        const uint64_t file_size = (uint64_t)end;
+        std::fseek(in, 0, SEEK_SET);
+        validate_stream(in, file_size);
+        std::fseek(in, 0, SEEK_SET);
         ck(vzCreate(0, &ctx), "vzCreate");
         vzServeConfig cfg{};
         cfg.hbm_budget_bytes = a.budget_mb << 20;
@@ -190,26 +236,10 @@ int main(int argc, char** argv)
         auto now_ms = [&] {
             return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         };
-        std::fseek(in, 0, SEEK_END);
-        const uint64_t file_size = (uint64_t)std::ftell(in);
-        std::fseek(in, 0, SEEK_SET);
         uint64_t offset = 0;
         std::vector<unsigned char> rec;
-        for (uint64_t index = 0;; index++) {
-            rec.resize(VZ_WIRE_REQUEST_HEADER);
-            const size_t got = std::fread(rec.data(), 1, VZ_WIRE_REQUEST_HEADER, in);
-            if (got == 0) break;
+        for (uint64_t index = 0; read_record(in, file_size, offset, index, rec); index++) {
             const std::string where = "record " + std::to_string(index);
-            if (got < VZ_WIRE_REQUEST_HEADER) throw Fail{where + ": truncated header"};
-            uint32_t inline_bytes = 0;  // offset 88 of the request header
-            std::memcpy(&inline_bytes, rec.data() + 88, 4);
-            offset += VZ_WIRE_REQUEST_HEADER;
-            if (inline_bytes > file_size - offset) throw Fail{where + ": inline scene runs past the end of the file"};
-            offset += inline_bytes;
-            rec.resize(VZ_WIRE_REQUEST_HEADER + (size_t)inline_bytes);
-            if (inline_bytes &&
-                std::fread(rec.data() + VZ_WIRE_REQUEST_HEADER, 1, inline_bytes, in) != inline_bytes)
-                throw Fail{where + ": truncated inline scene"};
             Job j;
             size_t sb = 0;
             if (vzWireDecodeRequest(rec.data(), rec.size(), &j.desc, &sb, nullptr) != VZ_SUCCESS)
