@@ -93,3 +93,19 @@ def test_renderer_box_scene_fp64(tmp_path):
     acc, _, st = O.render(s, m, oracle_params(w, h, spp, depth, cam, accum_fp64=True), bvh=O.BVH(s))
     assert np.array_equal(got, acc)
     assert f'"segments": {st.segments}' in line
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("buffer", ["zero_copy", "p2p"])
+def test_output_buffer_types_give_the_same_frame(tmp_path, buffer):
+    """HIPOutputBuffer's ZERO_COPY (mapped pinned host memory written by the
+    GPU) and HIP_P2P types hold the same 8-bit frame as HIP_DEVICE."""
+    args = ["--width", "64", "--height", "40", "--spp", "4", "--depth", "8", "--frames", "2"]
+    d = tmp_path / "device"
+    z = tmp_path / buffer
+    d.mkdir()
+    z.mkdir()
+    _run_cli(d, *args)
+    _run_cli(z, *args, "--buffer", buffer)
+    assert (z / "img.ppm").read_bytes() == (d / "img.ppm").read_bytes()
+    assert (z / "accum.raw").read_bytes() == (d / "accum.raw").read_bytes()

@@ -5,7 +5,11 @@
 //
 //   vz_render [--scene book|box] [--spheres N] [--seed S] [--width W]
 //             [--height H] [--spp S] [--depth D] [--frames F] [--fp64]
-//             [--ppm file] [--pfm file] [--accum file]
+//             [--buffer device|zero_copy|p2p] [--ppm file] [--pfm file]
+//             [--accum file]
+//
+// --buffer picks the HIPOutputBuffer type the frames are drawn into (the
+// reference picks CUDA_DEVICE when headless, Core/Core.cpp:251-252).
 //
 // Rows are stored bottom-up on the device (y = 0 is the bottom row, the GL
 // texture convention of Core.cpp:141): PPM output flips them, PFM (which is
@@ -22,7 +26,7 @@
 namespace {
 
 struct Args {
-    std::string scene = "book", ppm, pfm, accum;
+    std::string scene = "book", ppm, pfm, accum, buffer = "device";
     uint32_t spheres = 1000000, seed = 5489, width = 1200, height = 800, spp = 16, depth = 4, frames = 1;
     bool fp64 = false;
 };
@@ -51,6 +55,7 @@ Args parse(int argc, char** argv)
         else if (k == "--ppm") a.ppm = val();
         else if (k == "--pfm") a.pfm = val();
         else if (k == "--accum") a.accum = val();
+        else if (k == "--buffer") a.buffer = val();
         else {
             std::fprintf(stderr, "unknown option %s\n", k.c_str());
             std::exit(2);
@@ -84,7 +89,11 @@ int main(int argc, char** argv)
             camera = Camera(vec3(13.0f, 2.0f, 3.0f), 20.0f, aspect, 0.1f, 10.0f);
             camera.SetForward(vec3(0.0f, 0.0f, 0.0f) - vec3(13.0f, 2.0f, 3.0f));
         }
-        HIPOutputBuffer<uchar4> out(HIPOutputBufferType::HIP_DEVICE, (int32_t)a.width, (int32_t)a.height);
+        HIPOutputBufferType btype = HIPOutputBufferType::HIP_DEVICE;
+        if (a.buffer == "zero_copy") btype = HIPOutputBufferType::ZERO_COPY;
+        else if (a.buffer == "p2p") btype = HIPOutputBufferType::HIP_P2P;
+        else if (a.buffer != "device") throw Exception("--buffer must be device, zero_copy or p2p");
+        HIPOutputBuffer<uchar4> out(btype, (int32_t)a.width, (int32_t)a.height);
         unsigned long long segments = 0;
         double gpu_ms = 0.0;
         auto t2 = std::chrono::steady_clock::now();
