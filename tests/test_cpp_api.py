@@ -109,3 +109,22 @@ def test_output_buffer_types_give_the_same_frame(tmp_path, buffer):
     _run_cli(z, *args, "--buffer", buffer)
     assert (z / "img.ppm").read_bytes() == (d / "img.ppm").read_bytes()
     assert (z / "accum.raw").read_bytes() == (d / "accum.raw").read_bytes()
+
+
+@pytest.mark.gpu
+def test_camera_move_restarts_the_progressive_average(tmp_path, book):
+    """Renderer::Draw restarts the average when the camera changed
+    (Core/Renderer.h:37-54): frame 1 at the book camera, then the camera moves
+    and frames 2-3 must equal frames 1-2 rendered afresh from the new position."""
+    w, h, spp, depth = 64, 40, 4, 8
+    out, _ = _run_cli(tmp_path, "--width", str(w), "--height", str(h), "--spp", str(spp), "--depth", str(depth),
+                      "--frames", "3", "--move-after", "1", "--move-to", "10,3,6")
+    got = np.fromfile(out, np.float32).reshape(h, w, 4)
+    s, m = book
+    cam = dict(book_lookat(w, h))
+    cam["lookfrom"] = [10, 3, 6]
+    acc = None
+    b = O.BVH(s)
+    for f in (1, 2):
+        acc, _, _ = O.render(s, m, oracle_params(w, h, spp, depth, cam, subframe=f), bvh=b, accum=acc)
+    assert np.array_equal(got, acc)

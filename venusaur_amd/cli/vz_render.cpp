@@ -5,11 +5,14 @@
 //
 //   vz_render [--scene book|box] [--spheres N] [--seed S] [--width W]
 //             [--height H] [--spp S] [--depth D] [--frames F] [--fp64]
-//             [--buffer device|zero_copy|p2p] [--ppm file] [--pfm file]
-//             [--accum file]
+//             [--buffer device|zero_copy|p2p] [--move-after K --move-to X,Y,Z]
+//             [--ppm file] [--pfm file] [--accum file]
 //
 // --buffer picks the HIPOutputBuffer type the frames are drawn into (the
 // reference picks CUDA_DEVICE when headless, Core/Core.cpp:251-252).
+// --move-after K --move-to X,Y,Z moves the camera there (still looking at the
+// scene's target) after K frames, as the interactive loop does on input
+// (Core/Core.cpp:33-82): the next Draw restarts the progressive average.
 //
 // Rows are stored bottom-up on the device (y = 0 is the bottom row, the GL
 // texture convention of Core.cpp:141): PPM output flips them, PFM (which is
@@ -28,6 +31,8 @@ namespace {
 struct Args {
     std::string scene = "book", ppm, pfm, accum, buffer = "device";
     uint32_t spheres = 1000000, seed = 5489, width = 1200, height = 800, spp = 16, depth = 4, frames = 1;
+    uint32_t move_after = 0;
+    float move_to[3] = {0.0f, 0.0f, 0.0f};
     bool fp64 = false;
 };
 
@@ -56,6 +61,14 @@ Args parse(int argc, char** argv)
         else if (k == "--pfm") a.pfm = val();
         else if (k == "--accum") a.accum = val();
         else if (k == "--buffer") a.buffer = val();
+        else if (k == "--move-after") a.move_after = (uint32_t)std::stoul(val());
+        else if (k == "--move-to") {
+            const std::string v = val();
+            if (std::sscanf(v.c_str(), "%f,%f,%f", &a.move_to[0], &a.move_to[1], &a.move_to[2]) != 3) {
+                std::fprintf(stderr, "--move-to wants X,Y,Z\n");
+                std::exit(2);
+            }
+        }
         else {
             std::fprintf(stderr, "unknown option %s\n", k.c_str());
             std::exit(2);
@@ -97,7 +110,13 @@ int main(int argc, char** argv)
         unsigned long long segments = 0;
         double gpu_ms = 0.0;
         auto t2 = std::chrono::steady_clock::now();
+        const vec3 target(0.0f, 0.0f, 0.0f);
         for (uint32_t f = 0; f < a.frames; f++) {
+            if (a.move_after && f == a.move_after) {
+                const vec3 to(a.move_to[0], a.move_to[1], a.move_to[2]);
+                camera.SetPosition(to);
+                camera.SetForward(target - to);
+            }
             renderer.Draw(camera, out);
             segments += renderer.LastStats().segments;
             gpu_ms += renderer.LastStats().render_ms;
