@@ -261,3 +261,54 @@ def test_vz_serve_rejects_a_malformed_stream_before_any_gpu_work(tmp_path):
         r = _serve("--in", str(path))
         assert r.returncode == 1 and msg in r.stderr, (name, r.stderr)
         assert "vzCreate" not in r.stderr, name
+
+
+def _fuzz_base():
+    s, m = _small_scene()
+    d = api.request_desc(3, 4, 32, 24, 2, 4, [0, 1, -5], [0, 0, 0], 40.0, arrival_ms=2.5)
+    return api.encode_request(d, s, m)
+
+
+def test_fuzzed_records_decode_or_raise():
+    """Random byte edits, truncations and extensions of a valid request record
+    (and pure garbage): the decoders either return a record that passes the
+    same checks again on re-encoding, or raise VZ_ERROR_INVALID_VALUE; they
+    never crash or read past the buffer they are given (the buffer ctypes
+    hands over is exactly len(data) bytes)."""
+    from hypothesis import given, settings, HealthCheck
+    from hypothesis import strategies as st
+    base = _fuzz_base()
+
+    @settings(max_examples=1500, deadline=None, suppress_health_check=list(HealthCheck), derandomize=True)
+    @given(st.lists(st.tuples(st.integers(0, len(base) - 1), st.integers(0, 255)), max_size=6),
+           st.integers(0, len(base) + 16), st.binary(max_size=16))
+    def check(edits, cut, tail):
+        b = bytearray(base)
+        for pos, val in edits:
+            b[pos] = val
+        data = bytes(b[:cut]) + (tail if cut >= len(base) else b"")
+        try:
+            d, scene, used = api.decode_request(data)
+        except api.VzError as e:
+            assert e.status == 1
+            return
+        assert used <= len(data) and used == 96 + len(scene)
+        if scene:
+            s, m, n = api.decode_scene(scene)
+            assert n == len(scene)
+            assert api.encode_request(d, s, m) == data[:used]
+        else:
+            assert api.encode_request(d) == data[:used]
+
+    check()
+
+    @settings(max_examples=500, deadline=None, suppress_health_check=list(HealthCheck), derandomize=True)
+    @given(st.binary(max_size=400))
+    def garbage(data):
+        for fn in (api.decode_request, api.decode_scene):
+            try:
+                fn(data)
+            except api.VzError as e:
+                assert e.status == 1
+
+    garbage()
