@@ -25,7 +25,8 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_vz_serve_stream_matches_plain_launches(ctx, tmp_path):
+@pytest.mark.parametrize("realtime", [False, True])
+def test_vz_serve_stream_matches_plain_launches(ctx, tmp_path, realtime):
     assert os.path.exists(SERVE), "venusaur_amd/bin/vz_serve is missing: build it with __graft_entry__.build()"
     book = api.book_scene()
     box = api.box_scene(3000, 77)
@@ -49,9 +50,9 @@ def test_vz_serve_stream_matches_plain_launches(ctx, tmp_path):
     path.write_bytes(stream)
     out_dir = tmp_path / "accum"
     out_dir.mkdir()
-    say("case: vz_serve over 6 wire-format requests (book + 3000-sphere box scene)")
-    run = subprocess.run([SERVE, "--in", str(path), "--accum-dir", str(out_dir), "--budget-mb", "4096"],
-                         capture_output=True, text=True, timeout=120)
+    say(f"case: vz_serve over 6 wire-format requests (book + 3000-sphere box scene), realtime={realtime}")
+    cmd = [SERVE, "--in", str(path), "--accum-dir", str(out_dir), "--budget-mb", "4096"]
+    run = subprocess.run(cmd + (["--realtime"] if realtime else []), capture_output=True, text=True, timeout=120)
     say(run.stderr[-2000:])
     assert run.returncode == 1, run.stdout + run.stderr  # one request fails, by design
     lines = [json.loads(x) for x in run.stdout.strip().splitlines()]
@@ -61,6 +62,9 @@ def test_vz_serve_stream_matches_plain_launches(ctx, tmp_path):
     assert (summary["requests"], summary["ok"], summary["scenes_resident"]) == (6, 5, 2)
     assert (summary["inline_scenes_built"], summary["inline_scenes_ignored"]) == (2, 1)
     assert summary["p50_ms"] > 0 and summary["p99_ms"] >= summary["p50_ms"]
+    assert summary["realtime"] is realtime
+    if realtime:  # each request is submitted no earlier than its arrival offset
+        assert all(x["submit_ms"] >= x["arrival_ms"] for x in per if x["status"] == "VZ_SUCCESS")
     scenes = {k: api.Scene(ctx, s, m) for k, (s, m) in arrays.items()}
     try:
         for tag, d in enumerate(descs):
