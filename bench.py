@@ -51,6 +51,8 @@ def parse():
                     help="first copy MIB MiB device-to-device (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes in the batched-form comparison (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=120, help="requests in the serving comparison (0 = skip)")
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="batched form without autotuner or hipGraph, each step launch timed (for profiling)")
     ap.add_argument("--c2-frames", type=int, default=1,
                     help="timed frames of the 1M-sphere traversal-bound scene (0 = skip)")
     return ap.parse_args()
@@ -121,8 +123,11 @@ def _cheapest(reqs, k):
     return sorted(range(len(reqs)), key=lambda i: (reqs[i].paths * reqs[i].n_small))[:k]
 
 
-def run_batched(ctx, n_scenes, rmse_samples=2):
-    """Batched form vs sequential plain launches of the same 64-scene mix."""
+def run_batched(ctx, n_scenes, rmse_samples=2, eager=False):
+    """Batched form vs sequential plain launches of the same 64-scene mix.
+    eager: skip the autotuner and the hipGraph (default tuning, every step
+    launch timed with HIP events), the form rocprofv3 can profile; adds a
+    counting launch and the step kernel's algorithmic bytes."""
     from venusaur_amd import api, workloads
     from venusaur_amd.hipmem import synchronize
     reqs = workloads.batch_requests(n_scenes)
@@ -135,12 +140,34 @@ def run_batched(ctx, n_scenes, rmse_samples=2):
     segs_plain = sum(scenes[r.scene_key].launch(r.params).segments for r in reqs)
     t_plain = time.perf_counter() - t0
     want = [_sha(r.accum_host()) for r in reqs]
-    tuning = api.autotune(ctx, items)  # untimed: picks queue capacity and tile shape
+    if eager:
+        tuning = ctx.default_tuning()
+        for r in reqs:
+            r.params.flags |= api.TIME_STAGES
+        items = [(scenes[r.scene_key], r.params) for r in reqs]
+    else:
+        tuning = api.autotune(ctx, items)  # untimed: picks queue capacity and tile shape
     api.batch_launch(ctx, items, tuning=tuning)  # warm the engine and its graph
     synchronize()
     t0 = time.perf_counter()
     st = api.batch_launch(ctx, items, tuning=tuning)
     t_batch = time.perf_counter() - t0
+    step = None
+    if eager:
+        avg_ms = st.trace_ms / st.trace_launches
+        counted = []
+        for r in reqs:
+            c = api.LaunchParams.from_buffer_copy(r.params)
+            c.flags = (c.flags & ~api.TIME_STAGES) | api.COUNT_VISITS
+            counted.append((scenes[r.scene_key], c))
+        cst = api.batch_launch(ctx, counted, tuning=tuning)
+        per_launch = step_bytes(cst) / st.trace_launches
+        step = {"kernel": "k_wf_step (multi-scene)", "launches": st.trace_launches, "avg_launch_ms": round(avg_ms, 4),
+                "algorithmic_bytes_per_launch": round(per_launch),
+                "algorithmic_gbs": round(per_launch / (avg_ms * 1e-3) / 1e9, 1),
+                "node_visits_per_segment": round(cst.node_visits / cst.segments, 2),
+                "sphere_tests_per_segment": round(cst.sphere_tests / cst.segments, 2)}
+        step.update(_traffic("step_kernel_traffic_batch64.json", avg_ms))
     same = all(_sha(r.accum_host()) == w for r, w in zip(reqs, want))
     rmse64 = {f"item{k}_{reqs[k].width}px_{reqs[k].spp}spp_{reqs[k].n_small}sph": _rmse_fp64(reqs[k])
               for k in _cheapest(reqs, rmse_samples)}
@@ -151,6 +178,9 @@ def run_batched(ctx, n_scenes, rmse_samples=2):
            "batched_s": round(t_batch, 3), "bit_identical_to_plain": same, "segments_match": segs_plain == st.segments,
            "tuning": tuning.as_dict(), "lbvh_build_ms_mean": round(float(np.mean(build_ms)), 3),
            "rmse_vs_fp64_oracle": rmse64}
+    if eager:
+        out["mode"] = "eager (default tuning, no hipGraph, every step launch timed)"
+        out["step_kernel"] = step
     for sc in scenes.values():
         sc.close()
     return out
@@ -423,7 +453,7 @@ def main():
         forms["traversal_bound"] = run_c2(ctx, a.c2_frames)
     if a.batch_scenes:
         _progress(f"batched form ({a.batch_scenes} scenes)")
-        forms["batched"] = run_batched(ctx, a.batch_scenes)
+        forms["batched"] = run_batched(ctx, a.batch_scenes, eager=a.batch_eager)
     if a.serve_requests:
         _progress(f"served form ({a.serve_requests} requests)")
         cap = plain_capacity(ctx)
