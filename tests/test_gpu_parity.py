@@ -304,3 +304,38 @@ def test_context_outlives_its_scenes_and_servers():
     keep = api.Scene(c2, s, m)
     c2.close()
     assert not keep.h
+
+
+def test_two_contexts_on_two_threads(book):
+    """Separate contexts may be used from different threads: each thread
+    creates its own context and scene and renders; both results are
+    bit-exact against the oracle."""
+    import threading
+    say("case: two contexts on two threads")
+    s, m = book
+    jobs = [(64, 48, 4, 8), (48, 40, 6, 16)]
+    want = [O.render(s, m, oracle_params(w, h, spp, d, book_lookat(w, h)), bvh=O.BVH(s))[0] for w, h, spp, d in jobs]
+    got, errors = [None, None], []
+
+    def worker(k):
+        try:
+            w, h, spp, d = jobs[k]
+            c = api.Context(0)
+            sc = api.Scene(c, s, m)
+            for _ in range(3):  # several launches, interleaved with the other thread's
+                t = GpuTarget(w, h)
+                sc.launch(t.bind(gpu_params(w, h, spp, d, book_lookat(w, h))))
+                got[k] = t.accum_host()
+            sc.close()
+            c.close()
+        except Exception as e:  # reported by the main thread
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=worker, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errors, errors
+    for k in range(2):
+        assert np.array_equal(got[k], want[k]), k
