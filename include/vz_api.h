@@ -194,10 +194,14 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out);
  * the request completes.  Returns immediately with its id. */
 vzStatus vzServerSubmit(vzServer srv, vzScene scene, const vzLaunchParams* params,
                         uint64_t* request_id);
-/* Result of a finished request (VZ_ERROR_NOT_READY if still in flight). */
+/* Result of a finished request (VZ_ERROR_NOT_READY if still in flight).
+ * Results stay available, to any number of polls, until vzServerDestroy
+ * (about 48 bytes per request). */
 vzStatus vzServerPoll(vzServer srv, uint64_t request_id, vzRequestResult* out);
 /* Block until every submitted request finished. */
 vzStatus vzServerDrain(vzServer srv);
+/* Finishes every submitted request, then stops the loop and frees the
+ * server's engine and stream. */
 vzStatus vzServerDestroy(vzServer srv);
 /* Bytes of device memory a request of these params needs (admission). */
 uint64_t vzRequestFootprint(const vzLaunchParams* params);
