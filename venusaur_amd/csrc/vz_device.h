@@ -129,14 +129,27 @@ VZ_DEV bool hit_sphere(v3 o, v3 d, float tmin, float tmax, float4 sph, float& to
     float sq = sqrtf(disc);
     float q = -(hb + copysignf(sq, hb));
     if (q == 0.0f) return false;
-    float r1 = q / a;
-    float r2 = cc / q;
-    float tn = fminf(r1, r2);
-    float tf = fmaxf(r1, r2);
-    float t = tn;
-    if (t < tmin || tmax < t) {
-        t = tf;
-        if (t < tmin || tmax < t) return false;
+    // The roots are q/a and cc/q.  With hb < 0 (sign bit set) q > 0 and cc/q
+    // is the nearer; otherwise q < 0 and the nearer, q/a, is <= 0, which a
+    // positive tmin rejects without dividing.  The farther root is divided
+    // out only when the nearer one is rejected.
+    float t;
+    if (__builtin_signbit(hb)) {
+        t = cc / q;
+        if (t < tmin || tmax < t) {
+            t = q / a;
+            if (t < tmin || tmax < t) return false;
+        }
+    } else {
+        bool near_ok = false;
+        if (!(tmin > 0.0f)) {
+            t = q / a;
+            near_ok = !(t < tmin || tmax < t);
+        }
+        if (!near_ok) {
+            t = cc / q;
+            if (t < tmin || tmax < t) return false;
+        }
     }
     tout = t;
     return true;
