@@ -59,11 +59,12 @@ struct ReqState {            // device, advanced by the control kernel
     unsigned long long segments;  // segments of the passes accumulated so far
 };
 
-struct Counters {            // written by the control kernel only
+struct Counters {            // written by the control kernel, except fill (host, per run)
     uint32_t iteration;
     uint32_t active;         // requests still running
     uint32_t n_slots;        // request slots in use (table prefix)
     uint32_t total_regen;
+    uint32_t fill;           // rays a slice is filled to (0: its capacity R)
 };
 
 struct BlockStats {          // one per queue slice: its step workgroup writes hits..segments, the regen workgroups add paths atomically
@@ -153,6 +154,7 @@ struct RunOptions {
     bool use_graph = false;
     bool count_visits = false;
     bool time_trace = false;
+    uint64_t rays_in_flight = 0;  // cap on queued rays for this run (0: the whole queue)
 };
 
 struct RunResult {

@@ -327,11 +327,15 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     const uint32_t nslots = cnt->n_slots;
     const uint32_t iter = cnt->iteration;
     // 1. free room of the slices this thread owns (kSlicesPerThread * r + q)
+    // A run may cap the rays in flight below the queue's capacity: every
+    // slice is then filled to Rf <= R (slice counts never exceed Rf, since
+    // grants stop there and a step only removes rays).
+    const uint32_t Rf = (cnt->fill != 0u && cnt->fill < R) ? cnt->fill : R;
     uint32_t fr[kSlicesPerThread];
     uint32_t my_free = 0;
     for (uint32_t q = 0; q < kSlicesPerThread; q++) {
         const uint32_t sb = r * kSlicesPerThread + q;
-        fr[q] = sb < NB ? R - cnt_next[sb] : 0u;
+        fr[q] = (sb < NB && cnt_next[sb] < Rf) ? Rf - cnt_next[sb] : 0u;
         my_free += fr[q];
     }
     uint32_t freeq;
@@ -418,14 +422,14 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     }
     // 3. split the granted camera rays over the slices so that every slice
     // ends up near the same ray count: the next step launch lasts as long as
-    // its fullest slice.  Level L = ceil(rays after regen / NB) <= R; a slice
+    // its fullest slice.  Level L = ceil(rays after regen / NB) <= Rf; a slice
     // below L gets a share of the new rays proportional to its deficit.
-    const unsigned long long held = (unsigned long long)NB * R - freeq;
+    const unsigned long long held = (unsigned long long)NB * Rf - freeq;
     const uint32_t level = (uint32_t)((held + totR + NB - 1u) / NB);
     uint32_t def[kSlicesPerThread], my_def = 0;
     for (uint32_t q = 0; q < kSlicesPerThread; q++) {
-        const uint32_t c = R - fr[q];
-        def[q] = (r * kSlicesPerThread + q < NB && level > c) ? level - c : 0u;  // <= fr[q]: level <= R
+        const uint32_t c = Rf - fr[q];
+        def[q] = (r * kSlicesPerThread + q < NB && level > c) ? level - c : 0u;  // <= fr[q]: level <= Rf
         my_def += def[q];
     }
     uint32_t tdef;
@@ -456,7 +460,7 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     for (uint32_t q = 0; q < kSlicesPerThread; q++) {
         const uint32_t sb = r * kSlicesPerThread + q;
         if (sb < NB) {
-            const uint32_t c = R - fr[q];
+            const uint32_t c = Rf - fr[q];
             plan->blk_first[sb] = run;
             plan->blk_count[sb] = g[q];
             plan->blk_pos0[sb] = c;
@@ -909,6 +913,8 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
     std::memset(&c0, 0, sizeof(c0));
     c0.n_slots = n;
     c0.active = n;
+    if (opt.rays_in_flight)
+        c0.fill = (uint32_t)std::min<uint64_t>(e->R, std::max<uint64_t>(64, (opt.rays_in_flight + e->NB - 1) / e->NB));
     e->h_cnt[0] = c0;
     VZ_HIP_TRY(hipMemcpyAsync(e->d_cnt, &e->h_cnt[0], sizeof(Counters), hipMemcpyHostToDevice, st));
     VZ_HIP_TRY(hipMemsetAsync(e->d_bcnt, 0, 2 * e->NB * sizeof(uint32_t), st));
