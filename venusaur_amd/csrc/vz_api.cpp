@@ -383,13 +383,14 @@ vzStatus vzLaunch(vzScene scene, const vzLaunchParams* params, vzStream stream, 
     opt.use_graph = false;
     opt.count_visits = (params->flags & VZ_COUNT_VISITS) != 0;
     opt.time_trace = (params->flags & VZ_TIME_STAGES) != 0;
-    // Rays in flight for one request: about four image passes, at least 4M,
-    // at most the queue.  Measured on MI355X: requests up to 1024x1024 run
-    // fastest with ~4M rays in flight (a 120-request mix of the serving
-    // distribution: 2326 against 1964 Mray-segments/s with the full 8M
-    // queue), 1080p frames with the full 8M (book scene 11.17 against 10.54
-    // Gray-segments/s at 4M).
-    constexpr uint64_t kPassesInFlight = 4, kMinRaysInFlight = 4u << 20;
+    // Rays in flight for one request: four image passes, at least 1M, at
+    // most the queue.  Measured on MI355X (same box, plain launches one after
+    // another): a 120-request mix of the serving distribution 2345 against
+    // 1929 Mray-segments/s with the whole 8M queue in use, the 64-scene batch
+    // mix 2106 against 1998; 1080p frames fill the 8M queue either way (book
+    // scene 11.2 Gray-segments/s).  Six passes, or two and three (which also
+    // cut 1080p below 8M), were slower; floors from 256k to 1M were equal.
+    constexpr uint64_t kPassesInFlight = 4, kMinRaysInFlight = 1u << 20;
     opt.rays_in_flight =
         std::max<uint64_t>(kMinRaysInFlight, kPassesInFlight * (uint64_t)params->width * params->height);
     RunResult rr;
