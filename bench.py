@@ -261,7 +261,7 @@ def plain_capacity(ctx):
     return cap
 
 
-def run_served(ctx, n_requests, cap, load=3.0, rmse_samples=3):
+def run_served(ctx, n_requests, cap, load=3.0, rmse_samples=3, server_tuning=None):
     """Serving form vs one-at-a-time plain launches on the same Poisson stream,
     with arrivals at `load` times the plain capacity `cap` (requests/s)."""
     import threading
@@ -287,9 +287,10 @@ def run_served(ctx, n_requests, cap, load=3.0, rmse_samples=3):
     t_plain_end = time.perf_counter() - t_start
     want = [_sha(r.accum_host()) for r in reqs]
     # served: a client thread submits at the arrival times
-    # the server runs with the engine defaults (8M-ray queue): its latency
-    # depends on the queue size, which the batch autotuner does not weigh
-    srv = api.Server(ctx, hbm_budget_bytes=64 << 30, max_active=512, tuning=api.Tuning())
+    # the server runs with the serving defaults (8M-ray queue, 8x8 tiles): its
+    # latency depends on the queue size, which the batch autotuner does not weigh
+    srv = api.Server(ctx, hbm_budget_bytes=64 << 30, max_active=512,
+                     tuning=server_tuning if server_tuning is not None else api.Tuning())
     ids = [None] * len(reqs)
     t_srv0 = time.perf_counter()  # the server's clock starts at creation (same monotonic clock)
     t_start = time.perf_counter()

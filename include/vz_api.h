@@ -179,7 +179,7 @@ vzStatus vzDefaultTuning(vzContext ctx, vzTuning* out);
 typedef struct vzServeConfig {
     uint64_t hbm_budget_bytes;   /* admission budget for request buffers + scenes */
     uint32_t max_active;         /* max requests in the device table (<= 1024) */
-    vzTuning tuning;
+    vzTuning tuning;             /* 0 fields take the serving defaults: 8M-ray queue, 8x8 tiles */
 } vzServeConfig;
 
 typedef struct vzRequestResult {

@@ -209,6 +209,12 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     if (s->cfg.max_active == 0 || s->cfg.max_active > kMaxRequests) s->cfg.max_active = kMaxRequests;
     if (s->cfg.hbm_budget_bytes == 0) s->cfg.hbm_budget_bytes = ctx->hbm_bytes / 2;
     EngineConfig ec;
+    // Serving defaults: the 8M-ray queue and 8x8 camera tiles.  On the
+    // 120-request stream at 3x the plain capacity, 8x8 tiles served 2675
+    // against 2524 Mray-segments/s for 32x2 at the same p50 (0.55 s); 4M and
+    // 2M queues served about as fast with p50 at 2.5 s and 1.2 s.
+    ec.tile_w = 8;
+    ec.tile_h = 8;
     if (s->cfg.tuning.queue_capacity) ec.queue_capacity = s->cfg.tuning.queue_capacity;
     if (s->cfg.tuning.tile_w) ec.tile_w = s->cfg.tuning.tile_w;
     if (s->cfg.tuning.tile_h) ec.tile_h = s->cfg.tuning.tile_h;
