@@ -167,10 +167,10 @@ typedef struct vzTuning {
  * items share the ray/shade/miss queues.  Synchronises the stream. */
 vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n,
                        const vzTuning* tuning, vzStream stream, vzStats* stats);
-/* Renders the batch once per candidate tuning (queue capacity 4M/8M/12M rays
- * x camera tiles 32x2/8x8), timing the wavefront only, and returns the
- * fastest; caches the choice per context.  The items' accumulators are
- * written (as by vzBatchLaunch). */
+/* Renders the batch twice per candidate tuning (queue capacity 4M/8M/12M
+ * rays x camera tiles 32x2/8x8), timing the wavefront only, and returns the
+ * candidate with the fastest run; caches the choice per context.  The items'
+ * accumulators are written (as by vzBatchLaunch). */
 vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStream stream,
                     vzTuning* out);
 vzStatus vzDefaultTuning(vzContext ctx, vzTuning* out);
