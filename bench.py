@@ -419,6 +419,20 @@ def main():
     cpu = None
     checks = {"gpu_segments_per_step": stats[0].segments, "paths_per_step": stats[0].paths,
               "lbvh_build_ms": round(build_ms, 3)}
+    # the same frame with its host traffic: scene from host arrays (upload and
+    # LBVH build), render, 8-bit image back to host.  Reported beside value,
+    # which keeps inputs resident in HBM.
+    synchronize()
+    t0 = time.perf_counter()
+    scene2 = api.Scene(ctx, spheres, mats)
+    st2 = scene2.launch(p)
+    host_img = img.download(np.uint8, (H, W, 4))
+    t_pcie = time.perf_counter() - t0
+    scene2.close()
+    checks["pcie_inclusive"] = {"mray_segments_per_s": round(st2.segments / t_pcie / 1e6, 1),
+                                "frame_ms": round(t_pcie * 1e3, 2),
+                                "host_bytes": int(spheres.nbytes + mats.nbytes + host_img.nbytes),
+                                "what": "scene from host arrays (upload + LBVH build), one frame, uchar4 image to host"}
     if not a.no_cpu:
         _progress("CPU baseline (oracle) on sampled rows")
         import oracle as O
