@@ -55,7 +55,86 @@ def parse():
                     help="batched form without autotuner or hipGraph, each step launch timed (for profiling)")
     ap.add_argument("--c2-frames", type=int, default=1,
                     help="timed frames of the 1M-sphere traversal-bound scene (0 = skip)")
+    ap.add_argument("--summarize", metavar="KERNEL",
+                    help="no GPU: summarise rocprofv3 CSVs of profiled bench runs for KERNEL "
+                         "(e.g. 'k_wf_step<false, true, true>') into --out-md / --out-json")
+    ap.add_argument("--trace-csv", help="kernel_stats.csv of the --kernel-trace --stats run")
+    ap.add_argument("--fetch-csv", help="counter_collection.csv of the --pmc FETCH_SIZE run")
+    ap.add_argument("--write-csv", help="counter_collection.csv of the --pmc WRITE_SIZE run")
+    ap.add_argument("--l2-csv", help="counter_collection.csv of the --pmc TCC_HIT_sum TCC_MISS_sum run")
+    ap.add_argument("--out-md")
+    ap.add_argument("--out-json")
+    ap.add_argument("--commands", default="", help="the profiling commands, ' ;; '-separated, for the summary")
     return ap.parse_args()
+
+
+def _kernel_name(n):
+    import re
+    m = re.search(r"(k_\w+|__amd_\w+)(<[^>]*>)?", n)
+    return (m.group(1) + (m.group(2) or "")) if m else n[:40]
+
+
+def summarize(a):
+    """Summary of rocprofv3 CSVs for one kernel: the kernel-trace table, HBM
+    traffic per dispatch from FETCH_SIZE / WRITE_SIZE scaled by the counters'
+    reading for the run's own device-to-device copy of --calibrate-copy MiB,
+    and the L2 hit rate (TCC_HIT / TCC_MISS).  Writes the markdown summary and
+    the JSON that bench.py reads (traffic_bytes_per_launch, l2_hit_rate)."""
+    import csv
+    import statistics
+    kern = a.summarize
+    rows = list(csv.DictReader(open(a.trace_csv)))
+    table = ["| kernel | calls | total ms | avg us | min us | max us | % |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        table.append(f"| {_kernel_name(r['Name'])} | {r['Calls']} | {int(r['TotalDurationNs']) / 1e6:.3f} | "
+                     f"{float(r['AverageNs']) / 1e3:.2f} | {int(r['MinNs']) / 1e3:.2f} | {int(r['MaxNs']) / 1e3:.2f} | "
+                     f"{float(r['Percentage']):.1f} |")
+    avg_ns = float({_kernel_name(r["Name"]): r for r in rows}[kern]["AverageNs"])
+
+    def counter(path, name=None):
+        vals = {"copy": [], "kern": []}
+        for r in csv.DictReader(open(path)):
+            if name and r["Counter_Name"] != name:
+                continue
+            k = _kernel_name(r["Kernel_Name"])
+            if k == "__amd_rocclr_copyBuffer":
+                vals["copy"].append(float(r["Counter_Value"]))
+            elif k == kern:
+                vals["kern"].append(float(r["Counter_Value"]))
+        return vals
+
+    f, w = counter(a.fetch_csv), counter(a.write_csv)
+    true_kib = a.calibrate_copy * 1024
+    ffac, wfac = f["copy"][0] / true_kib, w["copy"][0] / true_kib
+    kf, kw = statistics.mean(f["kern"]), statistics.mean(w["kern"])
+    fetch_b, write_b = kf * 1024 / ffac, kw * 1024 / wfac
+    md = [f"# rocprofv3 summary: {kern}", "", "Commands (run from the repository root on the MI355X box):", "",
+          "```", *[c.strip() for c in a.commands.split(" ;; ") if c.strip()], "```", "",
+          "## Kernel time (kernel trace, --stats)", "", *table, "",
+          "## HBM traffic of the dominant kernel (PMC)", "",
+          f"- calibration: a {a.calibrate_copy} MiB device-to-device copy (`__amd_rocclr_copyBuffer`) reads "
+          f"FETCH_SIZE = {f['copy'][0]:.1f} KiB and WRITE_SIZE = {w['copy'][0]:.1f} KiB, i.e. factors {ffac:.4f} "
+          f"(fetch) and {wfac:.4f} (write) of the true bytes; kernel counters below are divided by these factors.",
+          f"- `{kern}`: {len(f['kern'])} dispatches; mean FETCH_SIZE {kf:.1f} KiB, WRITE_SIZE {kw:.1f} KiB per dispatch",
+          f"- scaled HBM traffic per dispatch: fetch {fetch_b / 1e6:.3f} MB + write {write_b / 1e6:.3f} MB = "
+          f"{(fetch_b + write_b) / 1e6:.3f} MB",
+          f"- mean dispatch duration (kernel trace): {avg_ns / 1e3:.2f} us -> measured HBM rate "
+          f"{(fetch_b + write_b) / avg_ns:.1f} GB/s"]
+    out = {}
+    if a.l2_csv:
+        h, m = counter(a.l2_csv, "TCC_HIT_sum")["kern"], counter(a.l2_csv, "TCC_MISS_sum")["kern"]
+        hit, miss = statistics.mean(h), statistics.mean(m)
+        out = {"l2_hit_rate": round(hit / (hit + miss), 4), "l2_requests_per_launch": round(hit + miss)}
+        md += ["", "## L2 of the dominant kernel (PMC: TCC_HIT_sum, TCC_MISS_sum)", "",
+               f"- `{kern}`: {len(h)} dispatches; mean {hit:.4g} hits and {miss:.4g} misses per dispatch: hit rate "
+               f"{hit / (hit + miss):.3f}, {(hit + miss) / (avg_ns * 1e-9) / 1e9:.1f} G requests/s"]
+    open(a.out_md, "w").write("\n".join(md) + "\n")
+    out.update({"kernel": kern, "traffic_bytes_per_launch": round(fetch_b + write_b),
+                "fetch_bytes_per_launch": round(fetch_b), "write_bytes_per_launch": round(write_b),
+                "dispatches": len(f["kern"]), "avg_launch_ns_trace": avg_ns, "fetch_factor": ffac,
+                "write_factor": wfac, "source": os.path.relpath(a.out_md, ROOT)})
+    json.dump(out, open(a.out_json, "w"), indent=1)
 
 
 def _sha(a):
@@ -338,6 +417,9 @@ def run_served(ctx, n_requests, cap, load=3.0, rmse_samples=3, server_tuning=Non
 
 def main():
     a = parse()
+    if a.summarize:
+        summarize(a)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is design only, see DESIGN.md)"}))
