@@ -25,8 +25,11 @@ struct TraceResult {
 };
 
 // Where the 32-byte nodes come from: the scene's array in HBM, or a copy a
-// workgroup staged in LDS (small scenes).
+// workgroup staged in LDS (small scenes).  kLeafLoop picks how a visit's two
+// leaf children are tested (see Traversal::leaves), per node source, as
+// measured on the 1080p book scene (LDS) and 1M-sphere box scenes (HBM).
 struct GlobalNodes {
+    static constexpr bool kLeafLoop = false;
     const uint4* p;
     VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b) const
     {
@@ -35,6 +38,7 @@ struct GlobalNodes {
     }
 };
 struct LdsNodes {
+    static constexpr bool kLeafLoop = true;
     const uint4* p;  // points into __shared__ memory
     VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b) const
     {
@@ -217,14 +221,28 @@ struct Traversal {
 
     // Sphere tests of the pending visit's leaf children, in child order.  Both
     // children's spheres come in one load: pg + 1 < n holds after a visit that
-    // did not end the traversal.
+    // did not end the traversal.  LOOP: one copy of the test run once per
+    // pending leaf, so a wave whose lanes hold leaves at different child
+    // positions runs it once, not twice (faster with LDS nodes); otherwise a
+    // copy per child (faster with HBM nodes, whose waits it overlaps).
+    template <bool LOOP>
     VZ_DEV void leaves(const SceneView& sc, uint32_t& ns)
     {
         if ((pend & 3u) == 0u) return;
         float4 s0, s1;
         ldg2(sc.spheres + pg, s0, s1);
-        if (pend & 1u) test_leaf(sc, pg, s0, ns);
-        if (pend & 2u) test_leaf(sc, pg + 1u, s1, ns);
+        if (LOOP) {
+            uint32_t p = pend & 3u;
+#pragma clang loop unroll(disable)
+            do {
+                const bool first = (p & 1u) != 0u;
+                test_leaf(sc, first ? pg : pg + 1u, first ? s0 : s1, ns);
+                p &= p - 1u;
+            } while (p != 0u);
+        } else {
+            if (pend & 1u) test_leaf(sc, pg, s0, ns);
+            if (pend & 2u) test_leaf(sc, pg + 1u, s1, ns);
+        }
     }
 
     // Next node after the pending visit: the near internal child (the far one
@@ -284,7 +302,7 @@ struct Traversal {
                      uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns)
     {
         if (visit(sc, nodes, nv)) return true;
-        leaves(sc, ns);
+        leaves<NodeSrc::kLeafLoop>(sc, ns);
         return decide(sc, lds_node, lds_t, stride, ov);
     }
 };
