@@ -161,6 +161,9 @@ typedef struct vzTuning {
     uint32_t refill_lanes;    /* a wave fetches new rays once fewer than this many of its lanes
                                  still trace: 1 = only when all are done, 65 = after every
                                  node step, 0 = the default for the scene's node placement */
+    uint32_t slice_split;     /* queue slices per resident step workgroup: 1, 2 or 4 (0 = 4).
+                                 Slices of uneven cost (many scenes in one run) balance better
+                                 when split; vzLaunch always uses 1 */
 } vzTuning;
 
 /* Renders every item in one persistent wavefront captured as a hipGraph;

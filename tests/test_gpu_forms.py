@@ -77,20 +77,44 @@ def test_batched_alone_equals_mix(ctx, scenes, oracle_results):
         assert np.array_equal(t.accum_host(), want), MIX[k]
 
 
-@pytest.mark.parametrize("q,tw,th", [(1 << 16, 8, 8), (1 << 20, 16, 4), (1 << 22, 64, 1)])
-def test_batched_any_tuning_same_result(ctx, scenes, oracle_results, q, tw, th):
-    say(f"case: batched tuning q={q} tile={tw}x{th}")
+@pytest.mark.parametrize("q,tw,th,split", [(1 << 16, 8, 8, 4), (1 << 16, 8, 8, 1), (1 << 20, 16, 4, 2),
+                                           (1 << 22, 64, 1, 4), (1 << 23, 32, 2, 1)])
+def test_batched_any_tuning_same_result(ctx, scenes, oracle_results, q, tw, th, split):
+    say(f"case: batched tuning q={q} tile={tw}x{th} split={split}")
     items, tgts = _items(scenes)
     t = api.Tuning()
-    t.queue_capacity, t.tile_w, t.tile_h, t.iters_per_graph = q, tw, th, 8
+    t.queue_capacity, t.tile_w, t.tile_h, t.iters_per_graph, t.slice_split = q, tw, th, 8, split
     api.batch_launch(ctx, items, tuning=t)
     for k, (tg, (want, _)) in enumerate(zip(tgts, oracle_results)):
         assert np.array_equal(tg.accum_host(), want), MIX[k]
 
 
-def test_serving_continuous_batching(ctx, scenes, oracle_results):
-    say("case: serving, staggered submissions")
-    srv = api.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64)
+def test_plain_batch_plain_on_one_context(ctx, scenes, oracle_results):
+    """Plain launches use one queue slice per resident step workgroup and
+    batches four; the engine re-partitions its queue between runs."""
+    say("case: plain, batched (split 4), plain again on one context")
+    items, tgts = _items(scenes)
+    for rnd in range(2):
+        for k, ((sc, p), tg) in enumerate(zip(items, tgts)):
+            tg.accum.zero()
+            sc.launch(p)
+            assert np.array_equal(tg.accum_host(), oracle_results[k][0]), (rnd, MIX[k])
+        if rnd == 0:
+            for tg in tgts:
+                tg.accum.zero()
+            api.batch_launch(ctx, items)
+            for k, (tg, (want, _)) in enumerate(zip(tgts, oracle_results)):
+                assert np.array_equal(tg.accum_host(), want), MIX[k]
+
+
+@pytest.mark.parametrize("split", [0, 1])
+def test_serving_continuous_batching(ctx, scenes, oracle_results, split):
+    say(f"case: serving, staggered submissions, slice split {split or 'default'}")
+    tuning = None
+    if split:
+        tuning = api.Tuning()
+        tuning.slice_split = split
+    srv = api.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64, tuning=tuning)
     items, tgts = _items(scenes)
     ids = []
     for k, (sc, p) in enumerate(items):
@@ -137,6 +161,7 @@ def test_autotune_picks_a_candidate_and_keeps_results(ctx, scenes, oracle_result
     t = api.autotune(ctx, items)
     assert t.queue_capacity in (1 << 22, 1 << 23, 3 << 22)
     assert (t.tile_w, t.tile_h) in ((32, 2), (8, 8))
+    assert t.slice_split == 4
     for tg in tgts:
         tg.accum.zero()
     api.batch_launch(ctx, items, tuning=t)

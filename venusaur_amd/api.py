@@ -67,7 +67,7 @@ class BatchItem(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("queue_capacity", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32),
-                ("iters_per_graph", C.c_uint32), ("refill_lanes", C.c_uint32)]
+                ("iters_per_graph", C.c_uint32), ("refill_lanes", C.c_uint32), ("slice_split", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -423,6 +423,7 @@ vzStatus vzDefaultTuning(vzContext ctx, vzTuning* out)
     out->tile_h = cfg.tile_h;
     out->iters_per_graph = cfg.iters_per_graph;
     out->refill_lanes = cfg.refill_lanes;
+    out->slice_split = kBatchSplit;
     return VZ_SUCCESS;
 }
 
@@ -478,6 +479,7 @@ vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n, cons
     }
     opt.count_visits = count;
     opt.time_trace = timing;
+    opt.slice_split = t.slice_split ? t.slice_split : kBatchSplit;
     RunResult rr;
     VZ_TRY(engine_run(ctx->engine, run, as_stream(stream), opt, &rr));
     if (stats) {

@@ -30,6 +30,11 @@
 namespace vz {
 
 constexpr uint32_t kMaxBlocks = 4096;   // step-kernel workgroups (queue slices)
+constexpr uint32_t kMaxSplit = 4;       // queue slices per resident step workgroup, at most
+// Batches and servers split the queue 4 ways per resident step workgroup: on
+// the 64-scene batch 3828 against 2735 Mray-segments/s for 1 (2: 3429, 3:
+// 3556); plain launches of one scene keep 1 (1M box scene 2434 against 2259).
+constexpr uint32_t kBatchSplit = 4;
 constexpr uint32_t kDeadRay = 0xFFFFFFFFu;  // meta of a padding item: skipped
 
 struct ReqDesc {             // device table entry, written by the host
@@ -104,7 +109,9 @@ struct Engine {
     vzContext ctx = nullptr;
     EngineConfig cfg;
     uint32_t Q = 0;                        // rays in flight (NB * R)
-    uint32_t NB = 0, R = 0;                // slices and slice capacity
+    uint32_t NB = 0, R = 0;                // slices and slice capacity of the current partition
+    uint32_t NB_res = 0, R_base = 0;       // one slice per resident step workgroup (split 1)
+    uint32_t split = 1;                    // current partition: NB = NB_res * split
     float4* q_o[2] = {nullptr, nullptr};   // origin xyz, pixel
     float4* q_d[2] = {nullptr, nullptr};   // dir xyz, seed
     float4* q_t[2] = {nullptr, nullptr};   // throughput xyz, req<<16 | slot<<8 | depth
@@ -155,6 +162,7 @@ struct RunOptions {
     bool count_visits = false;
     bool time_trace = false;
     uint64_t rays_in_flight = 0;  // cap on queued rays for this run (0: the whole queue)
+    uint32_t slice_split = 1;     // queue slices per resident step workgroup (1, 2 or 4)
 };
 
 struct RunResult {
@@ -172,7 +180,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
 // ---- incremental interface for the serving loop -------------------------
 // Prepares the engine for incremental use (empty table, stream-ordered
 // request buffers, scene table capacity `max_scenes`).
-vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes);
+vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes, uint32_t slice_split);
 // Installs one request into a free slot (written to the device tables in
 // stream order, so it joins the wavefront at the next iteration).
 vzStatus engine_admit(Engine* e, const RunItem& item, hipStream_t st, uint32_t* slot);

@@ -708,10 +708,16 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
                                                      kStepBlk, 0) == hipSuccess && occ_lds >= 1)
         occ = std::min(occ, occ_lds);
     uint32_t cus = ctx->num_cus ? ctx->num_cus : 256;
-    e->NB = std::min<uint32_t>(kMaxBlocks, cus * (uint32_t)occ);
-    uint32_t want = std::max<uint32_t>(cfg.queue_capacity, e->NB * 64u);
-    e->R = (uint32_t)align_up((want + e->NB - 1) / e->NB, 64);
+    // one slice per resident step workgroup; R_base is a multiple of
+    // 64 * kMaxSplit so that every split keeps 64-ray slices
+    e->NB_res = std::min<uint32_t>(kMaxBlocks, cus * (uint32_t)occ);
+    uint32_t want = std::max<uint32_t>(cfg.queue_capacity, e->NB_res * 64u * kMaxSplit);
+    e->R_base = (uint32_t)align_up((want + e->NB_res - 1) / e->NB_res, 64 * kMaxSplit);
+    e->NB = e->NB_res;
+    e->R = e->R_base;
+    e->split = 1;
     e->Q = e->NB * e->R;
+    const size_t nb_max = (size_t)e->NB_res * kMaxSplit;
     e->cfg.queue_capacity = e->Q;
     size_t Q = e->Q;
     for (int i = 0; i < 2; i++) {
@@ -720,8 +726,8 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
             hipMalloc(&e->q_t[i], Q * sizeof(float4)) != hipSuccess)
             return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: ray queue allocation failed");
     }
-    if (hipMalloc(&e->d_bcnt, 2 * e->NB * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&e->d_bstats, e->NB * sizeof(BlockStats)) != hipSuccess ||
+    if (hipMalloc(&e->d_bcnt, 2 * nb_max * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&e->d_bstats, nb_max * sizeof(BlockStats)) != hipSuccess ||
         hipMalloc(&e->d_req, kMaxRequests * sizeof(ReqDesc)) != hipSuccess ||
         hipMalloc(&e->d_state, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
         hipMalloc(&e->d_cnt, sizeof(Counters)) != hipSuccess ||
@@ -730,7 +736,7 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
     if (hipHostMalloc(&e->h_cnt, 2 * sizeof(Counters)) != hipSuccess ||
         hipHostMalloc(&e->h_state, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
         hipHostMalloc(&e->h_req, kMaxRequests * sizeof(ReqDesc)) != hipSuccess ||
-        hipHostMalloc(&e->h_bstats, e->NB * sizeof(BlockStats)) != hipSuccess)
+        hipHostMalloc(&e->h_bstats, nb_max * sizeof(BlockStats)) != hipSuccess)
         return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: pinned allocation failed");
     std::memset(e->h_cnt, 0, 2 * sizeof(Counters));
     std::memset(e->h_state, 0, kMaxRequests * sizeof(ReqState));
@@ -738,8 +744,8 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
     if (hipMemset(e->d_state, 0, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
         hipMemset(e->d_cnt, 0, sizeof(Counters)) != hipSuccess ||
         hipMemset(e->d_plan, 0, sizeof(Plan)) != hipSuccess ||
-        hipMemset(e->d_bcnt, 0, 2 * e->NB * sizeof(uint32_t)) != hipSuccess ||
-        hipMemset(e->d_bstats, 0, e->NB * sizeof(BlockStats)) != hipSuccess)
+        hipMemset(e->d_bcnt, 0, 2 * nb_max * sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(e->d_bstats, 0, nb_max * sizeof(BlockStats)) != hipSuccess)
         return fail(VZ_ERROR_HIP, "engine: initial memset failed");
     e->grid_small = cus * 4;
     e->slot_mem.resize(kMaxRequests);
@@ -773,13 +779,13 @@ static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool
     hipLaunchKernelGGL(k_wf_control, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt, e->d_plan,
                        cnt_nxt, e->NB, e->R);
     hipLaunchKernelGGL(k_wf_apply, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_req, e->d_plan);
-    hipLaunchKernelGGL(k_wf_regen, dim3(kRegenOver * e->NB), dim3(kRegenBlk), 0, st, e->d_req, e->d_plan, e->NB, e->q_o[n],
+    hipLaunchKernelGGL(k_wf_regen, dim3(kRegenOver * e->NB_res), dim3(kRegenBlk), 0, st, e->d_req, e->d_plan, e->NB, e->q_o[n],
                        e->q_d[n], e->q_t[n], e->d_bstats, e->R);
 }
 
 static vzStatus ensure_graph(Engine* e, bool count, bool lds, bool single)
 {
-    const int key = (count ? 1 : 0) | (lds ? 2 : 0) | (single ? 4 : 0);
+    const int key = (count ? 1 : 0) | (lds ? 2 : 0) | (single ? 4 : 0) | (int)(e->split << 3);
     if (e->graph && e->graph_key == key) return VZ_SUCCESS;
     if (e->graph) {
         (void)hipGraphExecDestroy(e->graph);
@@ -885,6 +891,20 @@ static vzStatus upload_scenes(Engine* e, const std::vector<const vzScene_st*>& s
     return VZ_SUCCESS;
 }
 
+// Partitions the queue into NB_res * f slices of R_base / f rays (f = 1, 2 or
+// 4, halved while NB would exceed kMaxBlocks).  More slices than resident
+// step workgroups let the dispatcher balance slices of uneven cost (many
+// scenes in one run); one per workgroup costs least when every ray costs
+// about the same.
+static void set_split(Engine* e, uint32_t f)
+{
+    f = f >= 4u ? 4u : (f >= 2u ? 2u : 1u);
+    while (f > 1u && (size_t)e->NB_res * f > kMaxBlocks) f >>= 1;
+    e->split = f;
+    e->NB = e->NB_res * f;
+    e->R = e->R_base / f;
+}
+
 vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st, const RunOptions& opt,
                     RunResult* res)
 {
@@ -905,6 +925,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
             scene_idx[i] = (uint32_t)(it - scenes.begin());
         }
     }
+    set_split(e, opt.slice_split);
     VZ_TRY(upload_scenes(e, scenes, st));
     for (uint32_t i = 0; i < n; i++) VZ_TRY(setup_slot(e, i, scene_idx[i], items[i].params, i, st));
     VZ_HIP_TRY(hipMemcpyAsync(e->d_req, e->h_req, n * sizeof(ReqDesc), hipMemcpyHostToDevice, st));
@@ -1026,8 +1047,9 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
 }
 
 // ---- serving -----------------------------------------------------------------------
-vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes)
+vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes, uint32_t slice_split)
 {
+    set_split(e, slice_split);
     e->stream_alloc = true;
     e->slot_used.assign(kMaxRequests, 0);
     e->n_slots = 0;

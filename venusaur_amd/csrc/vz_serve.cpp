@@ -224,7 +224,8 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     vzStatus rs = engine_create(ctx, ec, &s->engine);
     if (rs == VZ_SUCCESS && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         rs = set_error(VZ_ERROR_HIP, "hipStreamCreateWithFlags failed");
-    if (rs == VZ_SUCCESS) rs = engine_serve_init(s->engine, s->stream, 4096);
+    const uint32_t split = s->cfg.tuning.slice_split ? s->cfg.tuning.slice_split : kBatchSplit;
+    if (rs == VZ_SUCCESS) rs = engine_serve_init(s->engine, s->stream, 4096, split);
     if (rs != VZ_SUCCESS) {
         if (s->engine) engine_destroy(s->engine);
         if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -323,6 +324,7 @@ vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStrea
             cand.tile_h = t[1];
             cand.iters_per_graph = 16;
             cand.refill_lanes = 0;
+            cand.slice_split = kBatchSplit;
             double rate = 0.0;
             for (int rep = 0; rep < 2; rep++) {
                 vzStats st{};
