@@ -170,8 +170,8 @@ typedef struct vzTuning {
  * items share the ray/shade/miss queues.  Synchronises the stream. */
 vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n,
                        const vzTuning* tuning, vzStream stream, vzStats* stats);
-/* Renders the batch twice per candidate tuning (queue capacity 4M/8M/12M
- * rays x camera tiles 32x2/8x8), timing the wavefront only, and returns the
+/* Renders the batch twice per candidate tuning (queue capacity 4M/8M/12M/24M
+ * rays x camera tiles 32x2/8x8, queue split 4), timing the wavefront only, and returns the
  * candidate with the fastest run; caches the choice per context.  The items'
  * accumulators are written (as by vzBatchLaunch). */
 vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStream stream,
@@ -182,7 +182,8 @@ vzStatus vzDefaultTuning(vzContext ctx, vzTuning* out);
 typedef struct vzServeConfig {
     uint64_t hbm_budget_bytes;   /* admission budget for request buffers + scenes */
     uint32_t max_active;         /* max requests in the device table (<= 1024) */
-    vzTuning tuning;             /* 0 fields take the serving defaults: 8M-ray queue, 8x8 tiles */
+    vzTuning tuning;             /* 0 fields take the serving defaults: 12M-ray queue, 8x8 tiles,
+                                    queue split 4 */
 } vzServeConfig;
 
 typedef struct vzRequestResult {

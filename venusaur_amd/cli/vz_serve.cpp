@@ -230,7 +230,7 @@ int main(int argc, char** argv)
         vzServeConfig cfg{};
         cfg.hbm_budget_bytes = a.budget_mb << 20;
         cfg.max_active = a.max_active;
-        // cfg.tuning stays zero: the serving defaults (8M-ray queue, 8x8 tiles)
+        // cfg.tuning stays zero: the serving defaults (12M-ray queue, 8x8 tiles, split 4)
         ck(vzServerCreate(ctx, &cfg, &srv), "vzServerCreate");
         const auto t0 = std::chrono::steady_clock::now();
         auto now_ms = [&] {

@@ -210,10 +210,12 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     if (s->cfg.max_active == 0 || s->cfg.max_active > kMaxRequests) s->cfg.max_active = kMaxRequests;
     if (s->cfg.hbm_budget_bytes == 0) s->cfg.hbm_budget_bytes = ctx->hbm_bytes / 2;
     EngineConfig ec;
-    // Serving defaults: the 8M-ray queue and 8x8 camera tiles.  On the
-    // 120-request stream at 3x the plain capacity, 8x8 tiles served 2675
-    // against 2524 Mray-segments/s for 32x2 at the same p50 (0.55 s); 4M and
-    // 2M queues served about as fast with p50 at 2.5 s and 1.2 s.
+    // Serving defaults: a 12M-ray queue, 8x8 camera tiles and the queue split
+    // 4 ways per resident step workgroup.  On the 120-request stream at 3x
+    // the plain capacity: 12M served 3810 Mray-segments/s (p50 0.28 s, p99
+    // 3.95 s) against 3631 (0.31 s, 3.83 s) for 8M and 3798 (0.29 s, 4.15 s)
+    // for 24M; 32x2 tiles at 8M served 3595.
+    ec.queue_capacity = 3u << 22;
     ec.tile_w = 8;
     ec.tile_h = 8;
     if (s->cfg.tuning.queue_capacity) ec.queue_capacity = s->cfg.tuning.queue_capacity;
@@ -312,7 +314,7 @@ vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStrea
     // favoured small queues that lost on the real batch.
     std::vector<vzBatchItem> probe(items, items + n);
     for (auto& it : probe) it.params.flags &= ~(uint32_t)(VZ_COUNT_VISITS | VZ_TIME_STAGES);
-    const uint32_t caps[] = {1u << 22, 1u << 23, 3u << 22};
+    const uint32_t caps[] = {1u << 22, 1u << 23, 3u << 22, 6u << 22};
     const uint32_t tiles[][2] = {{32, 2}, {8, 8}};
     vzTuning best{};
     double best_rate = -1.0;
