@@ -32,8 +32,9 @@ namespace vz {
 constexpr uint32_t kMaxBlocks = 4096;   // step-kernel workgroups (queue slices)
 constexpr uint32_t kMaxSplit = 4;       // queue slices per resident step workgroup, at most
 // Batches and servers split the queue 4 ways per resident step workgroup: on
-// the 64-scene batch 3828 against 2735 Mray-segments/s for 1 (2: 3429, 3:
-// 3556); plain launches of one scene keep 1 (1M box scene 2434 against 2259).
+// the 64-scene batch (12M queue, 8x8 tiles) 3833 Mray-segments/s against 3422
+// for 2 and 2712 for 1; plain launches of one scene keep 1 (1M box scene 2434
+// against 2259 at 4).
 constexpr uint32_t kBatchSplit = 4;
 constexpr uint32_t kDeadRay = 0xFFFFFFFFu;  // meta of a padding item: skipped
 
