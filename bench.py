@@ -423,7 +423,7 @@ def main():
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is design only, see DESIGN.md)"}))
+                                   "(sharding across GPUs is designed but not built)"}))
         sys.exit(2)
     if not (os.path.exists(os.path.join(ROOT, "venusaur_amd", "libvz.so"))
             and os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so"))):

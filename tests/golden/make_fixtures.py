@@ -26,7 +26,7 @@ import oracle as O  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 
 # Values computed from the reference's random.cuh (tea<4> then three rnd()).
-SURVEY_KAT = [
+RNG_KAT = [
     (0, 1, 1569133783, [0.592472672, 0.00853163004, 0.544892073], 1921744396),
     (1, 1, 1546757732, [0.0165778995, 0.761493087, 0.713731587], 968275741),
     (2, 1, 1720073555, [0.78959024, 0.126846135, 0.995700359], 4127123000),
@@ -52,7 +52,7 @@ def rng_kat():
     ref = O.ref_rng()
     rows = []
     import ctypes as C
-    for v0, v1, tea, rnds, state in SURVEY_KAT:
+    for v0, v1, tea, rnds, state in RNG_KAT:
         t = O.tea4(v0, v1)
         r, st = O.rnd_seq(t, 3)
         assert t == tea and st == state, (v0, v1)
