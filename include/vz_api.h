@@ -166,8 +166,8 @@ typedef struct vzTuning {
                                  when split; vzLaunch always uses 1 */
 } vzTuning;
 
-/* Renders every item in one persistent wavefront captured as a hipGraph;
- * items share the ray/shade/miss queues.  Synchronises the stream. */
+/* Renders every item in one wavefront run (its iterations replayed as
+ * hipGraphs); the items share the ray queue.  Synchronises the stream. */
 vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n,
                        const vzTuning* tuning, vzStream stream, vzStats* stats);
 /* Renders the batch twice per candidate tuning (queue capacity 4M/8M/12M/24M

@@ -2,12 +2,15 @@
 // (vzLaunch), the batched form (vzBatchLaunch) and the serving loop.
 //
 // Layout.  The ray queue (two parities) is split into NB block-private
-// slices of R rays, one per persistent workgroup of the step kernel.  A
-// workgroup only ever touches its own slice, so claiming work and compacting
-// survivors use LDS atomics and wave ballots, never a device-wide atomic
-// (on MI355X a device-scope atomic to one address serialises across the 8
-// XCDs).  Each wavefront iteration runs four kernels on one stream:
-//   step    persistent: waves pull 64-ray chunks of their slice, trace them
+// slices of R rays, one per step-kernel workgroup: as many as fit on the
+// device at once (plain launches), or 4x that many smaller ones, which the
+// dispatcher hands out as workgroups finish (batches and servers, whose
+// slices differ in cost).  A workgroup only ever touches its own slice, so
+// claiming work and compacting survivors use LDS atomics and wave ballots,
+// never a device-wide atomic (on MI355X a device-scope atomic to one address
+// serialises across the 8 XCDs).  Each wavefront iteration runs four kernels
+// on one stream:
+//   step    one slice per workgroup: waves pull rays of their slice, trace them
 //           (LBVH traversal), run the miss stage (throughput * sky) or the
 //           shade stage (material scatter) in registers, and compact the
 //           surviving paths into the slice of the next parity
@@ -16,7 +19,7 @@
 //           the granted camera rays over the slices' free room
 //   apply   adds finished passes into per-pixel sums in sample order and
 //           writes accum/image when a request's last pass lands
-//   regen   each workgroup writes the camera rays granted to its slice
+//   regen   writes the granted camera rays into each slice's free room
 // A "pass" is sample s of every pixel of one request; its radiances land in
 // one of K ring slots, so per-pixel sums are formed in sample order whatever
 // order the paths finish in.  Iterations carry no host round trip; the host
