@@ -107,6 +107,38 @@ def test_plain_batch_plain_on_one_context(ctx, scenes, oracle_results):
                 assert np.array_equal(tg.accum_host(), want), MIX[k]
 
 
+def test_batch_at_the_request_table_limit(ctx, scenes):
+    """1024 items, the request table's size, in one batch with the queue split
+    4 and 1 ways: every accumulator equals a plain launch of the same item.
+    A 1025th item is refused with VZ_ERROR_CAPACITY."""
+    say("case: 1024-item batch (request-table limit), splits 4 and 1; 1025 refused")
+    names = ("book", "box2000")
+    items, tgts = [], []
+    for k in range(1024):
+        name = names[k % 2]
+        w, h = 8 + k % 5, 6 + k % 3
+        t = GpuTarget(w, h)
+        items.append((scenes[name][0], t.bind(gpu_params(w, h, 1 + k % 3, 4 + k % 4, _cam(name, w, h)))))
+        tgts.append(t)
+    want = []
+    for (sc, p), t in zip(items, tgts):
+        sc.launch(p)
+        want.append(t.accum_host())
+    for split in (4, 1):
+        for t in tgts:
+            t.accum.zero()
+        tu = api.Tuning()
+        tu.slice_split = split
+        api.batch_launch(ctx, items, tuning=tu)
+        for k, t in enumerate(tgts):
+            assert np.array_equal(t.accum_host(), want[k]), (split, k)
+    extra = GpuTarget(8, 6)
+    over = items + [(scenes["book"][0], extra.bind(gpu_params(8, 6, 1, 4, _cam("book", 8, 6))))]
+    with pytest.raises(api.VzError) as err:
+        api.batch_launch(ctx, over)
+    assert err.value.status == 5  # VZ_ERROR_CAPACITY
+
+
 @pytest.mark.parametrize("split", [0, 1])
 def test_serving_continuous_batching(ctx, scenes, oracle_results, split):
     say(f"case: serving, staggered submissions, slice split {split or 'default'}")
