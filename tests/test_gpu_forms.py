@@ -166,6 +166,33 @@ def test_serving_continuous_batching(ctx, scenes, oracle_results, split):
     srv.close()
 
 
+def test_serving_reuses_slots(ctx, scenes):
+    """300 small requests through a 64-entry table: slots are freed and
+    reused about five times over, and every result equals a plain launch."""
+    say("case: serving, 300 requests through 64 slots")
+    names = ("book", "box2000", "box12000")
+    items, tgts = [], []
+    for k in range(300):
+        name = names[k % 3]
+        w, h = 12 + k % 7, 9 + k % 4
+        t = GpuTarget(w, h)
+        items.append((scenes[name][0], t.bind(gpu_params(w, h, 1 + k % 4, 3 + k % 6, _cam(name, w, h)))))
+        tgts.append(t)
+    want = []
+    for (sc, p), t in zip(items, tgts):
+        sc.launch(p)
+        want.append(t.accum_host())
+        t.accum.zero()
+    srv = api.Server(ctx, hbm_budget_bytes=8 << 30, max_active=64)
+    ids = [srv.submit(sc, p) for sc, p in items]
+    srv.drain()
+    res = [srv.poll(i) for i in ids]
+    srv.close()
+    assert all(r is not None and r.status == 0 for r in res)
+    for k, t in enumerate(tgts):
+        assert np.array_equal(t.accum_host(), want[k]), k
+
+
 def test_serving_admission_by_footprint(ctx, scenes):
     say("case: serving admission budget")
     w, h, spp = 96, 64, 4
