@@ -26,6 +26,7 @@ constexpr uint32_t kRefillGlobal = 41;
 constexpr int kRegenBlk = 256;
 constexpr uint32_t kRegenOver = 8;  // regen workgroups per queue slice: ~one pixel run per thread at 1080p
 constexpr int kBlk = 256;
+constexpr uint32_t kAccBatch = 8;  // finished passes accumulated together when nothing waits on them
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
@@ -358,12 +359,21 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         npad = D.npad;
         uint32_t a = S.acc_next, c = 0;
         while (a + c < D.spp && (unsigned long long)(a + c + 1) * D.npad <= S.next_item &&
-               D.done[(a + c) % D.K] == D.npix) {
-            const uint32_t slot = (a + c) % D.K;
+               D.done[(a + c) % D.K] == D.npix)
+            c++;
+        // Fewer than kAccBatch finished passes wait for more, so that one
+        // apply reads several pass buffers and touches the sums once, unless
+        // this is the request's last pass or the ring room without them
+        // could bind a grant (it no longer covers the queue's free room).
+        // The sums are formed in sample order either way.
+        if (c != 0u && c < kAccBatch && a + c < D.spp &&
+            (unsigned long long)(a + D.K) * D.npad - S.next_item >= freeq)
+            c = 0u;
+        for (uint32_t q = 0; q < c; q++) {
+            const uint32_t slot = (a + q) % D.K;
             S.segments += D.segsum[slot];
             D.done[slot] = 0u;
             D.segsum[slot] = 0ull;
-            c++;
         }
         acc_first = a;
         acc_cnt = c;
