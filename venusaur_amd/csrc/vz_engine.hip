@@ -502,11 +502,11 @@ __device__ __forceinline__ uint32_t find_entry(const uint32_t* prefix, uint32_t 
 }
 
 // ---- apply: ordered per-pixel accumulation and the final blend/encode --------
-__global__ void __launch_bounds__(kBlk) k_wf_apply(const ReqDesc* __restrict__ req,
-                                                   const Plan* __restrict__ plan)
+__device__ __forceinline__ void apply_passes(const ReqDesc* __restrict__ req, const Plan* __restrict__ plan,
+                                             uint32_t bid, uint32_t nblk)
 {
     const uint32_t total = plan->total_acc_pix, n = plan->n_acc;
-    for (uint32_t j = blockIdx.x * kBlk + threadIdx.x; j < total; j += gridDim.x * kBlk) {
+    for (uint32_t j = bid * kBlk + threadIdx.x; j < total; j += nblk * kBlk) {
         uint32_t e = find_entry(plan->acc_prefix, n, j);
         const ReqDesc& D = req[plan->acc_req[e]];
         uint32_t px = j - plan->acc_prefix[e];
@@ -570,20 +570,17 @@ __global__ void __launch_bounds__(kBlk) k_wf_apply(const ReqDesc* __restrict__ r
 // a pixel's passes in order from a single seed read.  Item j of the grant
 // goes to granted index prefix + j, which the control kernel mapped to a
 // slice and a position in it.
-__global__ void __launch_bounds__(kRegenBlk) k_wf_regen(const ReqDesc* __restrict__ req,
-                                                        const Plan* __restrict__ plan, uint32_t NB,
-                                                        float4* __restrict__ n_o, float4* __restrict__ n_d,
-                                                        float4* __restrict__ n_t,
-                                                        BlockStats* __restrict__ bstats, uint32_t R)
+__device__ __forceinline__ void regen_rays(const ReqDesc* __restrict__ req, const Plan* __restrict__ plan,
+                                           uint32_t NB, float4* __restrict__ n_o, float4* __restrict__ n_d,
+                                           float4* __restrict__ n_t, BlockStats* __restrict__ bstats, uint32_t R,
+                                           uint32_t bid, uint32_t nblk, uint32_t* s_first, uint32_t* s_paths)
 {
-    __shared__ uint32_t s_first[kMaxBlocks];
-    __shared__ uint32_t s_paths[kRegenBlk / 64];
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < NB; i += kRegenBlk) s_first[i] = plan->blk_first[i];
     __syncthreads();
     const uint32_t nreg = plan->n_regen, runs = plan->total_runs;
     uint32_t paths = 0;
-    for (uint32_t r = blockIdx.x * kRegenBlk + tid; r < runs; r += gridDim.x * kRegenBlk) {
+    for (uint32_t r = bid * kRegenBlk + tid; r < runs; r += nblk * kRegenBlk) {
         const uint32_t e = find_entry(plan->run_prefix, nreg, r);
         const uint32_t rq = plan->regen_req[e];
         const ReqDesc& D = req[rq];
@@ -631,8 +628,30 @@ __global__ void __launch_bounds__(kRegenBlk) k_wf_regen(const ReqDesc* __restric
         unsigned long long p = 0;
         for (int w = 0; w < kRegenBlk / 64; w++) p += s_paths[w];
         // kRegenOver regen workgroups share each slice's stats entry
-        if (p) atomicAdd(&bstats[blockIdx.x % NB].paths, p);
+        if (p) atomicAdd(&bstats[bid % NB].paths, p);
     }
+}
+
+// ---- finish: apply and regen in one launch --------------------------------------
+// Both stages read the control kernel's plan and touch disjoint memory (apply:
+// finished passes' buffers, sums, accum and image; regen: the next queue
+// parity, camera seeds and path counts), so one launch runs both: the first
+// n_apply workgroups apply, the rest write camera rays, and the two overlap
+// instead of running back to back.
+static_assert(kBlk == kRegenBlk, "apply and regen share the workgroup size");
+__global__ void __launch_bounds__(kBlk) k_wf_finish(const ReqDesc* __restrict__ req,
+                                                    const Plan* __restrict__ plan, uint32_t NB,
+                                                    float4* __restrict__ n_o, float4* __restrict__ n_d,
+                                                    float4* __restrict__ n_t,
+                                                    BlockStats* __restrict__ bstats, uint32_t R, uint32_t n_apply)
+{
+    __shared__ uint32_t s_first[kMaxBlocks];
+    __shared__ uint32_t s_paths[kRegenBlk / 64];
+    if (blockIdx.x < n_apply)
+        apply_passes(req, plan, blockIdx.x, n_apply);
+    else
+        regen_rays(req, plan, NB, n_o, n_d, n_t, bstats, R, blockIdx.x - n_apply, gridDim.x - n_apply, s_first,
+                   s_paths);
 }
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -788,9 +807,8 @@ static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool
     if (ev1) (void)hipEventRecord(ev1, st);
     hipLaunchKernelGGL(k_wf_control, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt, e->d_plan,
                        cnt_nxt, e->NB, e->R);
-    hipLaunchKernelGGL(k_wf_apply, dim3(e->grid_small), dim3(kBlk), 0, st, e->d_req, e->d_plan);
-    hipLaunchKernelGGL(k_wf_regen, dim3(kRegenOver * e->NB_res), dim3(kRegenBlk), 0, st, e->d_req, e->d_plan, e->NB, e->q_o[n],
-                       e->q_d[n], e->q_t[n], e->d_bstats, e->R);
+    hipLaunchKernelGGL(k_wf_finish, dim3(e->grid_small + kRegenOver * e->NB_res), dim3(kBlk), 0, st, e->d_req,
+                       e->d_plan, e->NB, e->q_o[n], e->q_d[n], e->q_t[n], e->d_bstats, e->R, e->grid_small);
 }
 
 static vzStatus ensure_graph(Engine* e, bool count, bool lds, bool single)
