@@ -8,8 +8,8 @@
 // slices differ in cost).  A workgroup only ever touches its own slice, so
 // claiming work and compacting survivors use LDS atomics and wave ballots,
 // never a device-wide atomic (on MI355X a device-scope atomic to one address
-// serialises across the 8 XCDs).  Each wavefront iteration runs four kernels
-// on one stream:
+// serialises across the 8 XCDs).  Each wavefront iteration runs three launches
+// on one stream (the last with two stages on disjoint workgroups):
 //   step    one slice per workgroup: waves pull rays of their slice, trace them
 //           (LBVH traversal), run the miss stage (throughput * sky) or the
 //           shade stage (material scatter) in registers, and compact the
@@ -17,9 +17,9 @@
 //   control one workgroup: finds passes whose paths all ended, plans their
 //           accumulation, grants free queue space to requests and splits
 //           the granted camera rays over the slices' free room
-//   apply   adds finished passes into per-pixel sums in sample order and
-//           writes accum/image when a request's last pass lands
-//   regen   writes the granted camera rays into each slice's free room
+//   finish  apply: adds finished passes into per-pixel sums in sample order
+//           and writes accum/image when a request's last pass lands;
+//           regen: writes the granted camera rays into each slice's free room
 // A "pass" is sample s of every pixel of one request; its radiances land in
 // one of K ring slots, so per-pixel sums are formed in sample order whatever
 // order the paths finish in.  Iterations carry no host round trip; the host
