@@ -120,6 +120,8 @@ vzStatus vzSceneCreate(vzContext ctx, const vzSphere* spheres, const vzMaterial*
  * stream).  For callers that generate or keep scenes in HBM. */
 vzStatus vzSceneCreateDevice(vzContext ctx, const vzSphere* spheres, const vzMaterial* materials,
                              uint32_t n, vzStream stream, vzScene* out);
+/* Fails with VZ_ERROR_INVALID_VALUE while requests that name the scene are
+ * queued or running on a server (vzServerSubmit); the scene is unchanged. */
 vzStatus vzSceneDestroy(vzScene scene);
 /* Rebuild the LBVH from the current spheres. */
 vzStatus vzAccelBuild(vzScene scene, vzStream stream);
@@ -180,7 +182,10 @@ vzStatus vzDefaultTuning(vzContext ctx, vzTuning* out);
 
 /* ---- serving form --------------------------------------------------- */
 typedef struct vzServeConfig {
-    uint64_t hbm_budget_bytes;   /* admission budget for request buffers + scenes */
+    uint64_t hbm_budget_bytes;   /* admission budget: the per-request device buffers the server
+                                    allocates (vzRequestFootprint: pass ring, per-pixel sums,
+                                    camera seeds).  Scenes are the caller's and already resident;
+                                    they are not counted (vzAccelInfo.device_bytes gives theirs) */
     uint32_t max_active;         /* max requests in the device table (<= 1024) */
     vzTuning tuning;             /* 0 fields take the serving defaults: 12M-ray queue, 8x8 tiles,
                                     queue split 4 */
@@ -195,7 +200,8 @@ typedef struct vzRequestResult {
 
 vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out);
 /* Enqueue a request; the accum/image buffers in params must stay valid until
- * the request completes.  Returns immediately with its id. */
+ * the request completes.  Returns immediately with its id.  The scene is held
+ * until the request has finished: vzSceneDestroy refuses it meanwhile. */
 vzStatus vzServerSubmit(vzServer srv, vzScene scene, const vzLaunchParams* params,
                         uint64_t* request_id);
 /* Result of a finished request (VZ_ERROR_NOT_READY if still in flight).

@@ -10,6 +10,7 @@
 
 #include <math.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -941,7 +942,10 @@ static void run_job(job_t* j, vzo_stats* out)
     j->ntiles_y = (j->nrows + 15) / 16;
     j->next = 0;
     int nt = (int)p->threads;
-    if (nt <= 0) nt = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (nt <= 0) { /* the CPUs this process may run on, not every CPU of the host */
+        cpu_set_t cs;
+        nt = sched_getaffinity(0, sizeof(cs), &cs) == 0 ? CPU_COUNT(&cs) : (int)sysconf(_SC_NPROCESSORS_ONLN);
+    }
     if (nt < 1) nt = 1;
     if (nt > 256) nt = 256;
     memset(j->st, 0, sizeof(j->st));
@@ -1016,7 +1020,8 @@ void vzo_make_rayset(const vzo_bvh* b, const vzo_sphere* s, const vzo_material* 
     pctx_f32 c = {b, s, m, nsph, p->max_depth, out, 0, n_int};
     vzo_stats st;
     memset(&st, 0, sizeof(st));
-    for (uint32_t y = 0; y < p->height && c.rec_n < n_int; y++)
+    const uint32_t step = p->row_step ? p->row_step : 1u;
+    for (uint32_t y = p->row_start; y < p->height && c.rec_n < n_int; y += step)
         for (uint32_t x = 0; x < p->width && c.rec_n < n_int; x++) {
             float s32[3];
             double s64[3];

@@ -341,11 +341,14 @@ def test_depth_one_is_sky_or_black(book):
     p = _book_params(40, 24, 2, 1)
     acc, _, st = O.render(s, m, p, bvh=O.BVH(s))
     assert st.segments == st.paths  # one segment per path
-    rgb = acc[..., :3].reshape(-1, 3)
-    # every sample is either black (hit) or a sky colour (b = 1, r <= g <= 1)
-    sky_like = (rgb[:, 2] > 0) & (rgb[:, 0] <= rgb[:, 1] + 1e-6)
-    black = np.all(rgb == 0, axis=1)
-    assert np.all(sky_like | black | (rgb[:, 2] <= 1.0))
+    rgb = acc[..., :3].reshape(-1, 3).astype(np.float64)
+    # every sample is either black (its camera ray hit) or the sky colour of
+    # its direction, whose blue is exactly 1: so blue * spp counts the sky
+    # samples exactly, and r, g sit on the sky's lines through the same
+    # samples: B - R = 0.5 * sum(t) / spp and B - G = 0.3 * sum(t) / spp
+    b = rgb[:, 2] * 2
+    assert np.all(b == np.round(b)) and b.min() == 0 and b.max() == 2
+    assert np.allclose((rgb[:, 2] - rgb[:, 0]) / 0.5, (rgb[:, 2] - rgb[:, 1]) / 0.3, atol=1e-6)
     assert np.all(rgb <= 1.0 + 1e-6)
 
 

@@ -285,8 +285,15 @@ class Context:
 
     def close(self):
         if self.h:
-            for ch in list(self._children):
-                ch.close()
+            # servers first: they finish their requests and release the
+            # scenes those name, which can then be destroyed
+            children = list(self._children)
+            for ch in children:
+                if isinstance(ch, Server):
+                    ch.close()
+            for ch in children:
+                if not isinstance(ch, Server):
+                    ch.close()
             check(lib().vzDestroy(self.h))
             self.h = C.c_void_p()
 
@@ -335,8 +342,10 @@ class Scene:
         return self
 
     def close(self):
+        """vzSceneDestroy; raises (and keeps the scene) while server
+        requests that name it are queued or running."""
         if self.h:
-            lib().vzSceneDestroy(self.h)
+            check(lib().vzSceneDestroy(self.h))
             self.h = C.c_void_p()
 
     def __del__(self):
@@ -417,12 +426,16 @@ class Server:
         if tuning is not None:
             cfg.tuning = tuning
         self.h = C.c_void_p()
+        # scenes of requests not yet seen finished: kept alive here so that
+        # garbage collection cannot try to destroy a scene a request names
+        self._held = {}
         check(lib().vzServerCreate(ctx.h, C.byref(cfg), C.byref(self.h)))
         ctx._children.add(self)
 
     def submit(self, scene, params):
         rid = C.c_uint64()
         check(lib().vzServerSubmit(self.h, scene.h, C.byref(params), C.byref(rid)))
+        self._held[rid.value] = scene
         return rid.value
 
     def poll(self, rid):
@@ -431,15 +444,18 @@ class Server:
         if s == 4:
             return None
         check(s)
+        self._held.pop(rid, None)
         return r
 
     def drain(self):
         check(lib().vzServerDrain(self.h))
+        self._held.clear()
 
     def close(self):
         if self.h:
             lib().vzServerDestroy(self.h)
             self.h = C.c_void_p()
+            self._held.clear()
 
     def __del__(self):
         try:

@@ -57,7 +57,10 @@ public:
     }
 
     // Interactive moves (camera.h:36-78).  The reference rotates with
-    // glm's v * q, i.e. by -angle about the (normalised) axis.
+    // glm's v * q, i.e. by -angle about the (normalised) axis.  As there,
+    // the axes and the move directions are the frame of the last
+    // UpdateUVW (construction or UVWFrame): several moves between two
+    // frames all use that frame.
     void MoveForward(float speed)
     {
         m_position += m_forward * speed;
@@ -90,7 +93,6 @@ private:
     void Turn(const vec3& axis, float degrees)
     {
         m_forward = normalize(rotate(m_forward, normalize(axis), -degrees));
-        UpdateUVW();
         m_changed = true;
     }
 

@@ -14,6 +14,7 @@
 namespace vz {
 
 static thread_local std::string g_last_error;
+static std::atomic<uint64_t> g_next_scene_uid{1};
 
 vzStatus set_error(vzStatus s, const std::string& msg)
 {
@@ -163,6 +164,7 @@ vzStatus vzSceneCreate(vzContext ctx, const vzSphere* spheres, const vzMaterial*
     vzScene s = new vzScene_st();
     s->ctx = ctx;
     s->n = n;
+    s->uid = g_next_scene_uid.fetch_add(1);
     vzStatus rs = lbvh_alloc(s);
     hipStream_t st = as_stream(stream);
     if (rs == VZ_SUCCESS && (hipMemcpyAsync(s->d_sph_in, spheres, sizeof(vzSphere) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
@@ -190,6 +192,7 @@ vzStatus vzSceneCreateDevice(vzContext ctx, const vzSphere* spheres, const vzMat
     vzScene s = new vzScene_st();
     s->ctx = ctx;
     s->n = n;
+    s->uid = g_next_scene_uid.fetch_add(1);
     vzStatus rs = lbvh_alloc(s);
     hipStream_t st = as_stream(stream);
     if (rs == VZ_SUCCESS &&
@@ -218,6 +221,10 @@ vzStatus vzSceneCreateDevice(vzContext ctx, const vzSphere* spheres, const vzMat
 vzStatus vzSceneDestroy(vzScene scene)
 {
     if (!scene) return set_error(VZ_ERROR_INVALID_VALUE, "scene is NULL");
+    const uint32_t refs = scene->server_refs.load();
+    if (refs)
+        return set_error(VZ_ERROR_INVALID_VALUE, "scene is named by " + std::to_string(refs) +
+                                                     " queued or running server request(s)");
     (void)hipSetDevice(scene->ctx->device);
     (void)hipDeviceSynchronize();
     scene->ctx->live_scenes--;

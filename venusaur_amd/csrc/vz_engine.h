@@ -40,6 +40,12 @@ constexpr uint32_t kMaxSplit = 4;       // queue slices per resident step workgr
 // against 2259 at 4).
 constexpr uint32_t kBatchSplit = 4;
 constexpr uint32_t kDeadRay = 0xFFFFFFFFu;  // meta of a padding item: skipped
+// Scene table entries, allocated once when the engine is created, so that the
+// table's address never changes: captured hipGraphs bake it into their step
+// launches.  A batch names at most kMaxRequests distinct scenes; a server
+// holds at most max_active referenced ones and caches the rest until their
+// entries are needed.
+constexpr uint32_t kMaxScenes = 4096;
 
 struct ReqDesc {             // device table entry, written by the host
     Cam cam;
@@ -123,8 +129,7 @@ struct Engine {
     BlockStats* d_bstats = nullptr;        // NB partial stats
     ReqDesc* d_req = nullptr;
     ReqState* d_state = nullptr;
-    SceneView* d_scenes = nullptr;
-    uint32_t scene_cap = 0;
+    SceneView* d_scenes = nullptr;         // kMaxScenes entries, never reallocated
     Counters* d_cnt = nullptr;
     Plan* d_plan = nullptr;
     Counters* h_cnt = nullptr;             // pinned copies for polling (2)
@@ -133,9 +138,11 @@ struct Engine {
     SceneView* h_scenes = nullptr;
     BlockStats* h_bstats = nullptr;
     uint32_t grid_small = 0;
+    hipGraph_t graph_tmpl = nullptr;       // the captured graph, kept as long as its instance
     hipGraphExec_t graph = nullptr;
     hipStream_t graph_stream = nullptr;
     int graph_key = -1;
+    const SceneView* graph_scenes = nullptr;  // scene table the captured launches read
     // per-slot buffers, reused when large enough
     std::vector<DeviceBlock> slot_mem;
     uint64_t slot_bytes_total = 0;
@@ -147,7 +154,15 @@ struct Engine {
     ReqState* h_state_stage = nullptr;     // pinned staging for state uploads
     ReqState* h_status = nullptr;          // pinned, 2 x kMaxRequests status copies
     uint32_t* h_nslots = nullptr;          // pinned staging for the slot count
-    std::vector<const vzScene_st*> scene_list;
+    // serving scene table: entry i caches the device view of scene uid[i];
+    // refs counts the admitted requests that use it.  Entries with no refs
+    // stay cached for later requests of the same scene and are reused,
+    // least recently used first, when a new scene needs one.
+    std::vector<uint64_t> scene_uid;       // 0: never used
+    std::vector<uint32_t> scene_refs;
+    std::vector<uint64_t> scene_last_use;
+    std::vector<uint32_t> slot_scene;      // request slot -> scene entry
+    uint64_t scene_clock = 0;
 
     ~Engine();
 };
@@ -182,9 +197,9 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
                     const RunOptions& opt, RunResult* res);
 
 // ---- incremental interface for the serving loop -------------------------
-// Prepares the engine for incremental use (empty table, stream-ordered
-// request buffers, scene table capacity `max_scenes`).
-vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes, uint32_t slice_split);
+// Prepares the engine for incremental use (empty tables, stream-ordered
+// request buffers).
+vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t slice_split);
 // Installs one request into a free slot (written to the device tables in
 // stream order, so it joins the wavefront at the next iteration).
 vzStatus engine_admit(Engine* e, const RunItem& item, hipStream_t st, uint32_t* slot);
@@ -195,8 +210,11 @@ vzStatus engine_step(Engine* e, hipStream_t st, uint32_t k);
 // the engine_step that wrote it).
 void engine_slot_status(Engine* e, uint32_t k, uint32_t slot, bool* active,
                         unsigned long long* segments);
-// Frees a finished slot's buffers (stream-ordered) and marks it reusable.
+// Frees a finished slot's buffers (stream-ordered), drops its reference on
+// its scene entry and marks the slot reusable.
 vzStatus engine_release_slot(Engine* e, uint32_t slot, hipStream_t st);
+// Scene entries currently referenced by admitted requests (for tests).
+uint32_t engine_scenes_referenced(const Engine* e);
 size_t request_bytes(const vzLaunchParams& p);
 
 Cam make_cam(const vzLaunchParams& p);

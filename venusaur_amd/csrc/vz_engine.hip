@@ -699,6 +699,7 @@ size_t request_bytes(const vzLaunchParams& p)
 Engine::~Engine()
 {
     if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_tmpl) (void)hipGraphDestroy(graph_tmpl);
     if (graph_stream) (void)hipStreamDestroy(graph_stream);
     for (int i = 0; i < 2; i++) {
         if (q_o[i]) (void)hipFree(q_o[i]);
@@ -760,17 +761,21 @@ vzStatus engine_create(vzContext ctx, const EngineConfig& cfg, Engine** out)
         hipMalloc(&e->d_req, kMaxRequests * sizeof(ReqDesc)) != hipSuccess ||
         hipMalloc(&e->d_state, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
         hipMalloc(&e->d_cnt, sizeof(Counters)) != hipSuccess ||
-        hipMalloc(&e->d_plan, sizeof(Plan)) != hipSuccess)
+        hipMalloc(&e->d_plan, sizeof(Plan)) != hipSuccess ||
+        hipMalloc(&e->d_scenes, kMaxScenes * sizeof(SceneView)) != hipSuccess)
         return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: table allocation failed");
     if (hipHostMalloc(&e->h_cnt, 2 * sizeof(Counters)) != hipSuccess ||
         hipHostMalloc(&e->h_state, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
         hipHostMalloc(&e->h_req, kMaxRequests * sizeof(ReqDesc)) != hipSuccess ||
-        hipHostMalloc(&e->h_bstats, nb_max * sizeof(BlockStats)) != hipSuccess)
+        hipHostMalloc(&e->h_bstats, nb_max * sizeof(BlockStats)) != hipSuccess ||
+        hipHostMalloc(&e->h_scenes, kMaxScenes * sizeof(SceneView)) != hipSuccess)
         return fail(VZ_ERROR_OUT_OF_MEMORY, "engine: pinned allocation failed");
     std::memset(e->h_cnt, 0, 2 * sizeof(Counters));
     std::memset(e->h_state, 0, kMaxRequests * sizeof(ReqState));
     std::memset(e->h_req, 0, kMaxRequests * sizeof(ReqDesc));
+    std::memset(e->h_scenes, 0, kMaxScenes * sizeof(SceneView));
     if (hipMemset(e->d_state, 0, kMaxRequests * sizeof(ReqState)) != hipSuccess ||
+        hipMemset(e->d_scenes, 0, kMaxScenes * sizeof(SceneView)) != hipSuccess ||
         hipMemset(e->d_cnt, 0, sizeof(Counters)) != hipSuccess ||
         hipMemset(e->d_plan, 0, sizeof(Plan)) != hipSuccess ||
         hipMemset(e->d_bcnt, 0, 2 * nb_max * sizeof(uint32_t)) != hipSuccess ||
@@ -813,22 +818,42 @@ static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool
 
 static vzStatus ensure_graph(Engine* e, bool count, bool lds, bool single)
 {
+    // Every buffer the captured launches read is allocated once in
+    // engine_create; what varies between runs (the queue partition and the
+    // kernel variant) is in the key.
     const int key = (count ? 1 : 0) | (lds ? 2 : 0) | (single ? 4 : 0) | (int)(e->split << 3);
     if (e->graph && e->graph_key == key) return VZ_SUCCESS;
     if (e->graph) {
         (void)hipGraphExecDestroy(e->graph);
         e->graph = nullptr;
     }
+    if (e->graph_tmpl) {
+        (void)hipGraphDestroy(e->graph_tmpl);
+        e->graph_tmpl = nullptr;
+    }
+    e->graph_key = -1;
     if (!e->graph_stream) VZ_HIP_TRY(hipStreamCreateWithFlags(&e->graph_stream, hipStreamNonBlocking));
     VZ_HIP_TRY(hipStreamBeginCapture(e->graph_stream, hipStreamCaptureModeThreadLocal));
     for (uint32_t it = 0; it < e->cfg.iters_per_graph; it++)
         enqueue_iteration(e, e->graph_stream, (int)(it & 1u), count, lds, single, nullptr, nullptr);
-    hipGraph_t g = nullptr;
-    VZ_HIP_TRY(hipStreamEndCapture(e->graph_stream, &g));
-    hipError_t err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (err != hipSuccess) return set_error(VZ_ERROR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+    VZ_HIP_TRY(hipStreamEndCapture(e->graph_stream, &e->graph_tmpl));
+    hipError_t err = hipGraphInstantiate(&e->graph, e->graph_tmpl, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        e->graph = nullptr;
+        return set_error(VZ_ERROR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+    }
     e->graph_key = key;
+    e->graph_scenes = e->d_scenes;
+    return VZ_SUCCESS;
+}
+
+// Launches the captured iterations.  The launches read the scene table at the
+// address it had at capture, so check that it is still the engine's table.
+static vzStatus launch_graph(Engine* e, hipStream_t st)
+{
+    if (!e->graph || e->graph_scenes != e->d_scenes)
+        return set_error(VZ_ERROR_INTERNAL, "captured graph does not match the engine's scene table");
+    VZ_HIP_TRY(hipGraphLaunch(e->graph, st));
     return VZ_SUCCESS;
 }
 
@@ -904,16 +929,7 @@ static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const v
 static vzStatus upload_scenes(Engine* e, const std::vector<const vzScene_st*>& scenes, hipStream_t st)
 {
     uint32_t ns = (uint32_t)scenes.size();
-    if (ns > e->scene_cap) {
-        if (e->d_scenes) (void)hipFree(e->d_scenes);
-        if (e->h_scenes) (void)hipHostFree(e->h_scenes);
-        e->d_scenes = nullptr;
-        e->h_scenes = nullptr;
-        uint32_t cap = std::max<uint32_t>(ns, 64u);
-        VZ_HIP_TRY(hipMalloc(&e->d_scenes, cap * sizeof(SceneView)));
-        VZ_HIP_TRY(hipHostMalloc(&e->h_scenes, cap * sizeof(SceneView)));
-        e->scene_cap = cap;
-    }
+    if (ns > kMaxScenes) return set_error(VZ_ERROR_CAPACITY, "too many distinct scenes in one batch");
     for (uint32_t i = 0; i < ns; i++) e->h_scenes[i] = scenes[i]->view();
     VZ_HIP_TRY(hipMemcpyAsync(e->d_scenes, e->h_scenes, ns * sizeof(SceneView), hipMemcpyHostToDevice, st));
     return VZ_SUCCESS;
@@ -996,7 +1012,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
     uint32_t trace_launches = 0;
     auto launch_block = [&](uint32_t k) -> vzStatus {
         if (use_graph) {
-            VZ_HIP_TRY(hipGraphLaunch(e->graph, st));
+            VZ_TRY(launch_graph(e, st));
         } else {
             for (uint32_t it = 0; it < B; it++) {
                 hipEvent_t a = opt.time_trace ? tev[k & 1][2 * it].e : nullptr;
@@ -1075,24 +1091,18 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
 }
 
 // ---- serving -----------------------------------------------------------------------
-vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes, uint32_t slice_split)
+vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t slice_split)
 {
     set_split(e, slice_split);
     e->stream_alloc = true;
     e->slot_used.assign(kMaxRequests, 0);
+    e->slot_scene.assign(kMaxRequests, 0);
     e->n_slots = 0;
     e->next_seq = 0;
-    e->scene_list.clear();
-    max_scenes = std::max<uint32_t>(max_scenes, 1u);
-    if (e->scene_cap < max_scenes) {
-        if (e->d_scenes) (void)hipFree(e->d_scenes);
-        if (e->h_scenes) (void)hipHostFree(e->h_scenes);
-        e->d_scenes = nullptr;
-        e->h_scenes = nullptr;
-        VZ_HIP_TRY(hipMalloc(&e->d_scenes, max_scenes * sizeof(SceneView)));
-        VZ_HIP_TRY(hipHostMalloc(&e->h_scenes, max_scenes * sizeof(SceneView)));
-        e->scene_cap = max_scenes;
-    }
+    e->scene_uid.assign(kMaxScenes, 0);
+    e->scene_refs.assign(kMaxScenes, 0);
+    e->scene_last_use.assign(kMaxScenes, 0);
+    e->scene_clock = 0;
     if (!e->h_state_stage) VZ_HIP_TRY(hipHostMalloc(&e->h_state_stage, kMaxRequests * sizeof(ReqState)));
     if (!e->h_status) VZ_HIP_TRY(hipHostMalloc(&e->h_status, 2 * kMaxRequests * sizeof(ReqState)));
     if (!e->h_nslots) VZ_HIP_TRY(hipHostMalloc(&e->h_nslots, 64));
@@ -1106,6 +1116,33 @@ vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t max_scenes, uint3
     return VZ_SUCCESS;
 }
 
+// Scene entry for a request of `scene`: the cached one if the scene has one,
+// else an unused entry, else the least recently used entry that no admitted
+// request references.  An entry without references belongs to no ray in the
+// queue (a request is released only once all its paths have ended), so
+// rewriting its view in stream order is safe.
+static vzStatus scene_entry(Engine* e, const vzScene_st* scene, hipStream_t st, uint32_t* out)
+{
+    uint32_t pick = kMaxScenes;
+    uint64_t oldest = ~0ull;
+    for (uint32_t i = 0; i < kMaxScenes; i++) {
+        if (e->scene_uid[i] == scene->uid) {
+            *out = i;
+            return VZ_SUCCESS;
+        }
+        if (e->scene_refs[i] == 0u && e->scene_last_use[i] < oldest) {  // unused entries have last_use 0
+            oldest = e->scene_last_use[i];
+            pick = i;
+        }
+    }
+    if (pick == kMaxScenes) return set_error(VZ_ERROR_CAPACITY, "scene table full");
+    e->scene_uid[pick] = scene->uid;
+    e->h_scenes[pick] = scene->view();
+    VZ_HIP_TRY(hipMemcpyAsync(e->d_scenes + pick, e->h_scenes + pick, sizeof(SceneView), hipMemcpyHostToDevice, st));
+    *out = pick;
+    return VZ_SUCCESS;
+}
+
 vzStatus engine_admit(Engine* e, const RunItem& item, hipStream_t st, uint32_t* slot_out)
 {
     uint32_t slot = kMaxRequests;
@@ -1115,24 +1152,17 @@ vzStatus engine_admit(Engine* e, const RunItem& item, hipStream_t st, uint32_t* 
             break;
         }
     if (slot == kMaxRequests) return set_error(VZ_ERROR_CAPACITY, "request table full");
-    uint32_t sidx;
-    auto it = std::find(e->scene_list.begin(), e->scene_list.end(), item.scene);
-    if (it == e->scene_list.end()) {
-        if (e->scene_list.size() >= e->scene_cap) return set_error(VZ_ERROR_CAPACITY, "scene table full");
-        sidx = (uint32_t)e->scene_list.size();
-        e->scene_list.push_back(item.scene);
-        e->h_scenes[sidx] = item.scene->view();
-        VZ_HIP_TRY(hipMemcpyAsync(e->d_scenes + sidx, e->h_scenes + sidx, sizeof(SceneView), hipMemcpyHostToDevice,
-                                  st));
-    } else {
-        sidx = (uint32_t)(it - e->scene_list.begin());
-    }
+    uint32_t sidx = 0;
+    VZ_TRY(scene_entry(e, item.scene, st, &sidx));
     VZ_TRY(setup_slot(e, slot, sidx, item.params, e->next_seq++, st));
     e->h_state_stage[slot] = e->h_state[slot];
     VZ_HIP_TRY(hipMemcpyAsync(e->d_req + slot, e->h_req + slot, sizeof(ReqDesc), hipMemcpyHostToDevice, st));
     VZ_HIP_TRY(hipMemcpyAsync(e->d_state + slot, e->h_state_stage + slot, sizeof(ReqState), hipMemcpyHostToDevice,
                               st));
     e->slot_used[slot] = 1;
+    e->slot_scene[slot] = sidx;
+    e->scene_refs[sidx]++;
+    e->scene_last_use[sidx] = ++e->scene_clock;
     if (slot + 1 > e->n_slots) {
         e->n_slots = slot + 1;
         // the slot count only grows while serving, so a later value read by
@@ -1146,7 +1176,7 @@ vzStatus engine_admit(Engine* e, const RunItem& item, hipStream_t st, uint32_t* 
 
 vzStatus engine_step(Engine* e, hipStream_t st, uint32_t k)
 {
-    VZ_HIP_TRY(hipGraphLaunch(e->graph, st));
+    VZ_TRY(launch_graph(e, st));
     if (e->n_slots)
         VZ_HIP_TRY(hipMemcpyAsync(e->h_status + (k & 1u) * kMaxRequests, e->d_state, e->n_slots * sizeof(ReqState),
                                   hipMemcpyDeviceToHost, st));
@@ -1169,8 +1199,19 @@ vzStatus engine_release_slot(Engine* e, uint32_t slot, hipStream_t st)
         b.ptr = nullptr;
         b.bytes = 0;
     }
+    if (e->slot_used[slot] && !e->slot_scene.empty()) {
+        const uint32_t sidx = e->slot_scene[slot];
+        if (e->scene_refs[sidx]) e->scene_refs[sidx]--;
+    }
     e->slot_used[slot] = 0;
     return VZ_SUCCESS;
+}
+
+uint32_t engine_scenes_referenced(const Engine* e)
+{
+    uint32_t n = 0;
+    for (uint32_t r : e->scene_refs) n += r ? 1u : 0u;
+    return n;
 }
 
 }  // namespace vz

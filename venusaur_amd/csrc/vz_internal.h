@@ -98,6 +98,13 @@ struct vzContext_st {
 struct vzScene_st {
     vzContext ctx = nullptr;
     uint32_t n = 0;
+    // Unique over the process lifetime: tables that cache a scene's device
+    // view key it by this id, never by the object's address (a destroyed
+    // scene's address can be reused by the next one created).
+    uint64_t uid = 0;
+    // Server requests (queued or running) that name this scene; while any
+    // remain, vzSceneDestroy refuses.
+    std::atomic<uint32_t> server_refs{0};
     // inputs (original order)
     float4* d_sph_in = nullptr;
     vzMaterial* d_mat_in = nullptr;

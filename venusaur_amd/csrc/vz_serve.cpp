@@ -39,6 +39,7 @@ struct Pending {
 
 struct Active {
     uint64_t id;
+    const vzScene_st* scene;
     uint32_t slot;
     size_t bytes;
     uint32_t first_step;  // first engine_step whose status includes the slot
@@ -70,6 +71,11 @@ struct vzServer_st {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     void loop();
+    // A request no longer needs its scene (finished, rejected or dropped).
+    static void release_scene(const vzScene_st* sc)
+    {
+        const_cast<vzScene_st*>(sc)->server_refs.fetch_sub(1);
+    }
 };
 
 void vzServer_st::loop()
@@ -88,6 +94,7 @@ void vzServer_st::loop()
         cv_done.notify_all();
     };
     auto reject = [&](const Pending& p, uint32_t status) {
+        release_scene(p.scene);
         vzRequestResult r{};
         r.request_id = p.id;
         r.submit_ms = p.submit_ms;
@@ -134,6 +141,7 @@ void vzServer_st::loop()
                 }
                 Active a;
                 a.id = p.id;
+                a.scene = p.scene;
                 a.slot = slot;
                 a.bytes = p.bytes;
                 a.first_step = k;
@@ -175,6 +183,7 @@ void vzServer_st::loop()
                     fail(VZ_ERROR_HIP);
                     break;
                 }
+                release_scene(a.scene);
                 std::lock_guard<std::mutex> lk(mu);
                 vzRequestResult r{};
                 r.request_id = a.id;
@@ -227,7 +236,7 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     if (rs == VZ_SUCCESS && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         rs = set_error(VZ_ERROR_HIP, "hipStreamCreateWithFlags failed");
     const uint32_t split = s->cfg.tuning.slice_split ? s->cfg.tuning.slice_split : kBatchSplit;
-    if (rs == VZ_SUCCESS) rs = engine_serve_init(s->engine, s->stream, 4096, split);
+    if (rs == VZ_SUCCESS) rs = engine_serve_init(s->engine, s->stream, split);
     if (rs != VZ_SUCCESS) {
         if (s->engine) engine_destroy(s->engine);
         if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -252,6 +261,7 @@ vzStatus vzServerSubmit(vzServer srv, vzScene scene, const vzLaunchParams* param
     Pending p;
     p.id = srv->next_id++;
     p.scene = scene;
+    scene->server_refs.fetch_add(1);  // vzSceneDestroy refuses until the request is done
     p.params = *params;
     p.bytes = request_bytes(*params);
     p.submit_ms = srv->now_ms();
@@ -296,6 +306,11 @@ vzStatus vzServerDestroy(vzServer srv)
     }
     if (srv->worker.joinable()) srv->worker.join();
     (void)hipStreamSynchronize(srv->stream);
+    // after a fatal error requests may remain; they will never run
+    for (const Pending& p : srv->pending) vzServer_st::release_scene(p.scene);
+    for (const Active& a : srv->active) vzServer_st::release_scene(a.scene);
+    srv->pending.clear();
+    srv->active.clear();
     engine_destroy(srv->engine);
     (void)hipStreamDestroy(srv->stream);
     srv->ctx->live_servers--;
