@@ -33,6 +33,10 @@ import numpy as np  # noqa: E402
 
 METRIC = "Mray-segments/s @1080p; batched/served vs plain, p50/p99 latency, RMSE≤1e-3"
 HBM_PEAK_GBS = 8000.0
+# VALU lane throughput of MI355X for unpacked FP32/INT32 operations: 256 CUs x
+# 4 SIMDs x 16 lanes x 2.4 GHz (a wave64 instruction issues over 4 cycles);
+# the published 157.3 TFLOP/s FP32 is this x 2 (FMA) x 2 (packed pairs)
+VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
 
 
 def parse():
@@ -50,9 +54,15 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes in the batched-form comparison (0 = skip)")
-    ap.add_argument("--serve-requests", type=int, default=120, help="requests in the serving comparison (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests in the serving comparison at 3x the plain capacity (0 = skip)")
+    ap.add_argument("--serve-sustainable", type=int, default=0,
+                    help="requests in a second serving comparison at 0.8x the plain capacity (0 = skip)")
     ap.add_argument("--batch-eager", action="store_true",
                     help="batched form without autotuner or hipGraph, each step launch timed (for profiling)")
+    ap.add_argument("--batch-tuning", default="",
+                    help="QUEUE,TW,TH,SPLIT for --batch-eager (default: the context's default tuning), e.g. the "
+                         "autotuner's pick from an unprofiled run")
     ap.add_argument("--c2-frames", type=int, default=1,
                     help="timed frames of the 1M-sphere traversal-bound scene (0 = skip)")
     ap.add_argument("--summarize", metavar="KERNEL",
@@ -62,6 +72,11 @@ def parse():
     ap.add_argument("--fetch-csv", help="counter_collection.csv of the --pmc FETCH_SIZE run")
     ap.add_argument("--write-csv", help="counter_collection.csv of the --pmc WRITE_SIZE run")
     ap.add_argument("--l2-csv", help="counter_collection.csv of the --pmc TCC_HIT_sum TCC_MISS_sum run")
+    ap.add_argument("--summarize-sq", metavar="CSV",
+                    help="no GPU: summarise a rocprofv3 --pmc CSV of SQ counters (SQ_INSTS_VALU "
+                         "SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_BUSY_CYCLES ...) into --out-md and "
+                         "--out-json, for the kernels named by --sq-map")
+    ap.add_argument("--sq-map", default="", help="'KERNEL=key;KERNEL=key': workload key of each kernel")
     ap.add_argument("--out-md")
     ap.add_argument("--out-json")
     ap.add_argument("--commands", default="", help="the profiling commands, ' ;; '-separated, for the summary")
@@ -137,6 +152,43 @@ def summarize(a):
     json.dump(out, open(a.out_json, "w"), indent=1)
 
 
+def summarize_sq(a):
+    """Per-dispatch means of SQ counters for the kernels of --sq-map, with
+    the VALU issue fraction (instructions over the issue capacity of the
+    busy cycles: 1024 SIMDs, one wave64 VALU instruction per 4 cycles, the
+    busy-cycle counter summed over 32 shader engines) and the fraction of
+    lanes active per VALU instruction."""
+    import csv
+    import statistics
+    want = dict(x.split("=", 1) for x in a.sq_map.split(";") if x.strip())
+    vals = {}
+    for r in csv.DictReader(open(a.summarize_sq)):
+        k = _kernel_name(r["Kernel_Name"])
+        if k in want:
+            vals.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    out = json.load(open(a.out_json)) if os.path.exists(a.out_json) else {}
+    rows = ["| kernel | workload | dispatches | SQ_INSTS_VALU | SQ_ACTIVE_INST_VALU | SQ_THREAD_CYCLES_VALU | "
+            "SQ_BUSY_CYCLES | VALU issue / capacity | lanes active |", "|---|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for k, key in want.items():
+        v = {c: statistics.mean(x) for c, x in vals[k].items()}
+        n = len(vals[k]["SQ_INSTS_VALU"])
+        issue = v["SQ_INSTS_VALU"] / (v["SQ_BUSY_CYCLES"] / 32 * 1024 / 4)
+        lanes = v["SQ_THREAD_CYCLES_VALU"] / (64 * v["SQ_ACTIVE_INST_VALU"])
+        out[key] = {"kernel": k, "dispatches": n, "insts_valu": v["SQ_INSTS_VALU"],
+                    "active_inst_valu": v["SQ_ACTIVE_INST_VALU"], "thread_cycles_valu": v["SQ_THREAD_CYCLES_VALU"],
+                    "busy_cycles": v["SQ_BUSY_CYCLES"], "valu_issue_frac": round(issue, 3),
+                    "lanes_active": round(lanes, 3), "source": os.path.relpath(a.out_md, ROOT)}
+        rows.append(f"| `{k}` | {key} | {n} | {v['SQ_INSTS_VALU']:.4g} | {v['SQ_ACTIVE_INST_VALU']:.4g} | "
+                    f"{v['SQ_THREAD_CYCLES_VALU']:.4g} | {v['SQ_BUSY_CYCLES']:.4g} | {issue:.2f} | {lanes:.2f} |")
+    md = [f"# rocprofv3 SQ counters of the step kernels", "", "Command (run from the repository root on the MI355X box):",
+          "", "```", *[c.strip() for c in a.commands.split(" ;; ") if c.strip()], "```", "",
+          "Means per dispatch.  VALU issue capacity of a dispatch = SQ_BUSY_CYCLES / 32 shader engines x 1024 "
+          "SIMDs / 4 cycles per wave64 VALU instruction; lanes active = SQ_THREAD_CYCLES_VALU / "
+          "(64 x SQ_ACTIVE_INST_VALU).", "", *rows]
+    open(a.out_md, "w").write("\n".join(md) + "\n")
+    json.dump(out, open(a.out_json, "w"), indent=1)
+
+
 def _sha(a):
     import hashlib
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
@@ -172,6 +224,35 @@ def _traffic(name, avg_ms):
     return out
 
 
+def _sq(key):
+    """Per-dispatch SQ counters of a step kernel on a workload (committed
+    rocprofv3 --pmc summary, profiles/step_kernel_sq.json), or None."""
+    tf = os.path.join(ROOT, "profiles", "step_kernel_sq.json")
+    if not os.path.exists(tf):
+        return None
+    return json.load(open(tf)).get(key)
+
+
+def survey_bytes(st):
+    """The survey's algorithmic bytes per segment summed over a counting
+    launch: 32 B per node pair fetched, 16 B per sphere tested, 16 B material
+    per hit, and 144 B of queue traffic (ray record written and read, hit
+    record written and read, path state read and written)."""
+    return 32 * st.node_visits + 16 * st.sphere_tests + 16 * st.hits + 144 * st.segments
+
+
+def valu_roofline(sq, avg_ms):
+    """Useful VALU lane operations per second of a step kernel: the lanes
+    active over its VALU instructions per dispatch (SQ_THREAD_CYCLES_VALU, a
+    function of the workload) over the live mean dispatch time, against the
+    chip's lane throughput."""
+    achieved = sq["thread_cycles_valu"] / (avg_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TLANE, 2), "unit": "Tlane-op/s",
+            "frac": round(achieved / VALU_PEAK_TLANE, 4), "valu_issue_frac": sq["valu_issue_frac"],
+            "lanes_active": sq["lanes_active"], "lane_ops_per_launch": sq["thread_cycles_valu"],
+            "sq_source": sq["source"]}
+
+
 def _build_scenes(ctx, reqs):
     from venusaur_amd import api
     scenes, build_ms = {}, []
@@ -202,7 +283,7 @@ def _cheapest(reqs, k):
     return sorted(range(len(reqs)), key=lambda i: (reqs[i].paths * reqs[i].n_small))[:k]
 
 
-def run_batched(ctx, n_scenes, rmse_samples=2, eager=False):
+def run_batched(ctx, n_scenes, rmse_samples=2, eager=False, eager_tuning=""):
     """Batched form vs sequential plain launches of the same 64-scene mix.
     eager: skip the autotuner and the hipGraph (default tuning, every step
     launch timed with HIP events), the form rocprofv3 can profile; adds a
@@ -221,6 +302,9 @@ def run_batched(ctx, n_scenes, rmse_samples=2, eager=False):
     want = [_sha(r.accum_host()) for r in reqs]
     if eager:
         tuning = ctx.default_tuning()
+        if eager_tuning:
+            q, tw, th, sp = (int(x) for x in eager_tuning.split(","))
+            tuning.queue_capacity, tuning.tile_w, tuning.tile_h, tuning.slice_split = q, tw, th, sp
         for r in reqs:
             r.params.flags |= api.TIME_STAGES
         items = [(scenes[r.scene_key], r.params) for r in reqs]
@@ -258,7 +342,7 @@ def run_batched(ctx, n_scenes, rmse_samples=2, eager=False):
            "tuning": tuning.as_dict(), "lbvh_build_ms_mean": round(float(np.mean(build_ms)), 3),
            "rmse_vs_fp64_oracle": rmse64}
     if eager:
-        out["mode"] = "eager (default tuning, no hipGraph, every step launch timed)"
+        out["mode"] = "eager (no hipGraph, every step launch timed)"
         out["step_kernel"] = step
     for sc in scenes.values():
         sc.close()
@@ -313,8 +397,14 @@ def run_c2(ctx, frames, seed=2024):
            "sphere_tests_per_segment": round(cst.sphere_tests / cst.segments, 2),
            "step_kernel": {"launches": launches, "avg_launch_ms": round(avg_ms, 4),
                            "algorithmic_bytes_per_launch": round(per_launch),
-                           "algorithmic_gbs": round(per_launch / (avg_ms * 1e-3) / 1e9, 1)}}
+                           "algorithmic_gbs": round(per_launch / (avg_ms * 1e-3) / 1e9, 1),
+                           "algorithmic_frac_of_hbm": round(per_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "survey_formula_frac_of_hbm": round(survey_bytes(cst) * frames / launches
+                                                               / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     out["step_kernel"].update(_traffic("step_kernel_traffic_c2.json", avg_ms))
+    sq = _sq("box1M_1080p_64spp")
+    if sq:
+        out["step_kernel"]["valu"] = valu_roofline(sq, avg_ms)
     acc.free()
     img.free()
     sc.close()
@@ -421,6 +511,9 @@ def main():
     if a.summarize:
         summarize(a)
         return
+    if a.summarize_sq:
+        summarize_sq(a)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(sharding across GPUs is designed but not built)"}))
@@ -483,20 +576,43 @@ def main():
         per_launch_bytes = bytes_step * a.steps / launches
         avg_ms = trace_ms / launches
         achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+        full = (W, H, SPP, DEPTH) == (1920, 1080, 256, 50)
         # measured HBM bytes per launch of the same kernel on the same
         # workload: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy
         # kernel of known size (profiles/k_wf_step_book_1080p_256spp.md)
-        meas = {"traffic": None, "measured_hbm_gbs": None}
-        if (W, H, SPP, DEPTH) == (1920, 1080, 256, 50):
-            meas = _traffic("step_kernel_traffic.json", avg_ms)
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), **meas,
-                "algorithmic_bytes_per_launch": round(per_launch_bytes),
-                "kernel": "k_wf_step", "launches": launches, "avg_launch_ms": round(avg_ms, 4),
-                "trace_share_of_step": round(trace_ms / (sum(s.render_ms for s in stats)), 3),
-                "bytes_per_segment": round(bytes_step / seg_c, 1),
-                "node_visits_per_segment": round(nv_c / seg_c, 2),
-                "sphere_tests_per_segment": round(ns_c / seg_c, 2)}
+        meas = _traffic("step_kernel_traffic.json", avg_ms) if full else {"traffic": None, "measured_hbm_gbs": None}
+        # the book scene's nodes are staged in LDS, so its node reads (32 B
+        # per visit) never reach HBM: the HBM-side algorithmic bytes leave
+        # them out
+        lds_nodes = cst.node_visits if len(spheres) - 1 <= 512 else 0
+        hbm_alg = (bytes_step - 32 * lds_nodes) * a.steps / launches
+        svy = survey_bytes(cst) * a.steps / launches
+        hbm = {"peak_gbs": HBM_PEAK_GBS,
+               "step_bytes_per_launch": round(per_launch_bytes), "step_bytes_gbs": round(achieved, 1),
+               "step_bytes_frac": round(achieved / HBM_PEAK_GBS, 4),
+               "survey_formula_bytes_per_segment": round(survey_bytes(cst) / seg_c, 1),
+               "survey_formula_frac": round(svy / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               "node_bytes_from_lds_per_launch": round(32 * lds_nodes * a.steps / launches),
+               "hbm_algorithmic_bytes_per_launch": round(hbm_alg),
+               "hbm_algorithmic_frac": round(hbm_alg / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if meas.get("measured_hbm_gbs"):
+            hbm["measured_hbm_gbs"] = meas["measured_hbm_gbs"]
+            hbm["measured_hbm_frac"] = round(meas["measured_hbm_gbs"] / HBM_PEAK_GBS, 4)
+            for k in ("l2_hit_rate", "l2_requests_per_launch"):
+                if k in meas:
+                    hbm[k] = meas[k]
+        sq = _sq("book_1080p_256spp") if full else None
+        if sq:  # the step kernel issues VALU at capacity: that is its roof
+            roof = valu_roofline(sq, avg_ms)
+        else:
+            roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4)}
+        roof.update({"traffic": meas.get("traffic"), "kernel": "k_wf_step", "launches": launches,
+                     "avg_launch_ms": round(avg_ms, 4),
+                     "trace_share_of_step": round(trace_ms / (sum(s.render_ms for s in stats)), 3),
+                     "bytes_per_segment": round(bytes_step / seg_c, 1),
+                     "node_visits_per_segment": round(nv_c / seg_c, 2),
+                     "sphere_tests_per_segment": round(ns_c / seg_c, 2), "hbm": hbm})
 
     # CPU baseline: the oracle's FP32 integrator on every k-th row
     cpu = None
@@ -553,14 +669,17 @@ def main():
         forms["traversal_bound"] = run_c2(ctx, a.c2_frames)
     if a.batch_scenes:
         _progress(f"batched form ({a.batch_scenes} scenes)")
-        forms["batched"] = run_batched(ctx, a.batch_scenes, eager=a.batch_eager)
+        forms["batched"] = run_batched(ctx, a.batch_scenes, eager=a.batch_eager, eager_tuning=a.batch_tuning)
+    cap = None
     if a.serve_requests:
         _progress(f"served form ({a.serve_requests} requests)")
         cap = plain_capacity(ctx)
         forms["served"] = run_served(ctx, a.serve_requests, cap, load=3.0)
+    if a.serve_sustainable:
         # the same comparison below the plain capacity, where latency is not queue growth
-        _progress(f"served form at 0.8x the plain capacity ({a.serve_requests} requests)")
-        forms["served_sustainable"] = run_served(ctx, a.serve_requests, cap, load=0.8, rmse_samples=0)
+        _progress(f"served form at 0.8x the plain capacity ({a.serve_sustainable} requests)")
+        cap = cap or plain_capacity(ctx)
+        forms["served_sustainable"] = run_served(ctx, a.serve_sustainable, cap, load=0.8, rmse_samples=0)
 
     info = ctx.info()
     out = {

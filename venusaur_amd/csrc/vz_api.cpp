@@ -115,11 +115,16 @@ vzStatus vzCreate(int device, vzContext* out)
     VZ_HIP_TRY(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return set_error(VZ_ERROR_NO_DEVICE, std::string("libvz is built for gfx950, device is ") + prop.gcnArchName);
+    unsigned long long* scratch = nullptr;
+    VZ_TRY(trace_scratch_address(&scratch));  // this device's copy, looked up once with the device current
     vzContext c = new vzContext_st();
     c->device = device;
+    c->trace_scratch = scratch;
     c->num_cus = (uint32_t)prop.multiProcessorCount;
     c->hbm_bytes = (uint64_t)prop.totalGlobalMem;
-    c->name = prop.name;
+    // the marketing name can be empty (no entry in the driver's device-id
+    // table); the architecture name is always set
+    c->name = prop.name[0] ? std::string(prop.name) : std::string("AMD ") + prop.gcnArchName;
     *out = c;
     return VZ_SUCCESS;
 }
