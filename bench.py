@@ -233,12 +233,16 @@ def _sq(key):
     return json.load(open(tf)).get(key)
 
 
-def survey_bytes(st):
+NODE_BYTES = 64  # one 4-wide node: four quantised child boxes and the child words
+
+
+def survey_bytes(st, node_bytes=NODE_BYTES):
     """The survey's algorithmic bytes per segment summed over a counting
-    launch: 32 B per node pair fetched, 16 B per sphere tested, 16 B material
-    per hit, and 144 B of queue traffic (ray record written and read, hit
-    record written and read, path state read and written)."""
-    return 32 * st.node_visits + 16 * st.sphere_tests + 16 * st.hits + 144 * st.segments
+    launch: the node bytes fetched (a 4-wide node is 64 B, where the survey's
+    binary node pair was 32 B), 16 B per sphere tested, 16 B material per hit,
+    and 144 B of queue traffic (ray record written and read, hit record
+    written and read, path state read and written)."""
+    return node_bytes * st.node_visits + 16 * st.sphere_tests + 16 * st.hits + 144 * st.segments
 
 
 def valu_roofline(sq, avg_ms):
@@ -349,14 +353,14 @@ def run_batched(ctx, n_scenes, rmse_samples=2, eager=False, eager_tuning=""):
     return out
 
 
-def step_bytes(st):
+def step_bytes(st, node_bytes=NODE_BYTES):
     """Algorithmic bytes of the step kernel (trace + miss/shade stages) for a
-    counting launch: per segment the 48 B path record read, 32 B per BVH node
-    fetched, 16 B per sphere tested; per hit 36 B of sphere, material and type
-    re-read by the shade stage; per surviving path the 48 B record written to
-    the next queue; per ended path its 12 B radiance written to the pass
-    buffer."""
-    return (48 * st.segments + 32 * st.node_visits + 16 * st.sphere_tests + 36 * st.hits
+    counting launch: per segment the 48 B path record read, 64 B per 4-wide
+    BVH node fetched, 16 B per sphere tested; per hit 36 B of sphere, material
+    and type re-read by the shade stage; per surviving path the 48 B record
+    written to the next queue; per ended path its 12 B radiance written to the
+    pass buffer."""
+    return (48 * st.segments + node_bytes * st.node_visits + 16 * st.sphere_tests + 36 * st.hits
             + 48 * (st.segments - st.paths) + 12 * st.paths)
 
 
@@ -393,6 +397,7 @@ def run_c2(ctx, frames, seed=2024):
            "frames": frames, "value": round(segs / wall / 1e6, 2), "unit": "Mray-segments/s",
            "ms_per_frame": round(wall / frames * 1e3, 2), "segments_per_frame": int(stats[0].segments),
            "lbvh_build_ms": round(info.build_ms, 3), "bvh_depth": int(info.depth),
+           "wide_nodes": int(info.num_wide_nodes), "wide_depth": int(info.wide_depth),
            "host_generate_s": round(t_gen, 3), "node_visits_per_segment": round(cst.node_visits / cst.segments, 2),
            "sphere_tests_per_segment": round(cst.sphere_tests / cst.segments, 2),
            "step_kernel": {"launches": launches, "avg_launch_ms": round(avg_ms, 4),
@@ -566,9 +571,11 @@ def main():
     gpu_acc = acc.download(np.float32, (H, W, 4))
 
     # roofline of the step kernel (trace + miss/shade stages, the dominant
-    # kernel), from the algorithmic bytes of the counting launch
+    # kernel), from the algorithmic bytes of the counting launch; a scene of
+    # at most 513 spheres rendered alone walks its binary tree (32-byte nodes)
     seg_c, nv_c, ns_c = cst.segments, cst.node_visits, cst.sphere_tests
-    bytes_step = step_bytes(cst)
+    nb = 32 if len(spheres) - 1 <= 512 else NODE_BYTES
+    bytes_step = step_bytes(cst, nb)
     trace_ms = sum(s.trace_ms for s in stats)
     launches = sum(s.trace_launches for s in stats)
     roof = None
@@ -584,13 +591,15 @@ def main():
         # the book scene's nodes are staged in LDS, so its node reads (32 B
         # per visit) never reach HBM: the HBM-side algorithmic bytes leave
         # them out
+        # a scene of at most 513 spheres rendered alone walks its binary tree
+        # (32-byte nodes) staged in LDS, so those node reads never reach HBM
         lds_nodes = cst.node_visits if len(spheres) - 1 <= 512 else 0
         hbm_alg = (bytes_step - 32 * lds_nodes) * a.steps / launches
-        svy = survey_bytes(cst) * a.steps / launches
+        svy = survey_bytes(cst, nb) * a.steps / launches
         hbm = {"peak_gbs": HBM_PEAK_GBS,
                "step_bytes_per_launch": round(per_launch_bytes), "step_bytes_gbs": round(achieved, 1),
                "step_bytes_frac": round(achieved / HBM_PEAK_GBS, 4),
-               "survey_formula_bytes_per_segment": round(survey_bytes(cst) / seg_c, 1),
+               "survey_formula_bytes_per_segment": round(survey_bytes(cst, nb) / seg_c, 1),
                "survey_formula_frac": round(svy / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                "node_bytes_from_lds_per_launch": round(32 * lds_nodes * a.steps / launches),
                "hbm_algorithmic_bytes_per_launch": round(hbm_alg),

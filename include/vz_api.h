@@ -94,11 +94,13 @@ typedef struct vzStats {
 
 typedef struct vzAccelInfo {
     uint32_t num_prims;
-    uint32_t num_nodes;       /* internal nodes, num_prims - 1 */
-    uint32_t depth;           /* levels from the root to the deepest leaf */
-    uint32_t node_bytes;      /* bytes per node (32) */
+    uint32_t num_nodes;       /* binary internal nodes, num_prims - 1 */
+    uint32_t depth;           /* binary levels from the root to the deepest leaf */
+    uint32_t node_bytes;      /* bytes per binary node (32) */
     uint64_t device_bytes;    /* resident scene + BVH footprint */
     double build_ms;          /* last build/refit time (HIP events) */
+    uint32_t num_wide_nodes;  /* nodes of the 4-wide tree traversal uses (64 bytes each) */
+    uint32_t wide_depth;      /* its levels from the root to the deepest leaf */
 } vzAccelInfo;
 
 /* ---- context -------------------------------------------------------- */
@@ -138,6 +140,14 @@ vzStatus vzAccelInfoGet(vzScene scene, vzAccelInfo* out);
  * Any pointer may be NULL. */
 vzStatus vzAccelDownload(vzScene scene, void* nodes, uint32_t* leaf_prims, uint32_t* morton,
                          vzStream stream);
+/* Copies the 4-wide tree that every traversal walks (num_wide_nodes * 64
+ * bytes; synchronises the stream).  It is collapsed from the binary tree on
+ * every build and refit.  Node layout, 16 words: the node box's lo corner
+ * (3 floats); meta = per-axis scale exponents (bytes 0-2), the slots' leaf
+ * flags (bits 24-27) and valid flags (bits 28-31); the quantised lo planes
+ * of the 4 slots for x, y, z (byte c = slot c), then the hi planes; the 4
+ * children (leaf index or wide node index); 2 zero words. */
+vzStatus vzAccelDownloadWide(vzScene scene, void* wide_nodes, vzStream stream);
 
 /* ---- tracing and rendering ----------------------------------------- */
 /* Closest hit for n rays (device arrays).  Equivalent of one optixTrace per

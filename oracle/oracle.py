@@ -19,6 +19,8 @@ RAY_DT = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"),
                    ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
 HIT_DT = np.dtype([("t", "<f4"), ("prim", "<u4")])
 NODE_DT = np.dtype([("o", "<f4", 3), ("meta", "<u4"), ("q", "<u4", 3), ("child", "<u4")])
+WNODE_DT = np.dtype([("o", "<f4", 3), ("meta", "<u4"), ("qlo", "<u4", 3), ("qhi", "<u4", 3),
+                     ("child", "<u4", 4), ("pad", "<u4", 2)])
 MISS = 0xFFFFFFFF
 
 
@@ -31,7 +33,7 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("subframe", C.c_uint32), ("max_depth", C.c_uint32), ("cam", Camera),
                 ("row_start", C.c_uint32), ("row_step", C.c_uint32), ("threads", C.c_uint32),
-                ("accum_fp64", C.c_uint32)]
+                ("accum_fp64", C.c_uint32), ("binary_tree", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -73,6 +75,11 @@ def lib():
             "vzo_bvh_morton": (vp, [vp]),
             "vzo_bvh_node_boxes": (vp, [vp]),
             "vzo_bvh_depth": (u32, [vp]),
+            "vzo_bvh_num_wide_nodes": (u32, [vp]),
+            "vzo_bvh_wide_nodes": (vp, [vp]),
+            "vzo_bvh_wide_depth": (u32, [vp]),
+            "vzo_trace_binary": (None, [vp, vp, vp, u32, vp, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64)]),
             "vzo_bvh_validate": (C.c_int, [vp, vp]),
             "vzo_trace": (None, [vp, vp, vp, u32, vp, C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_uint64)]),
@@ -191,15 +198,31 @@ class BVH:
     def depth(self):
         return lib().vzo_bvh_depth(self.h)
 
+    def wide_nodes(self):
+        return self._arr(lib().vzo_bvh_wide_nodes, WNODE_DT, lib().vzo_bvh_num_wide_nodes(self.h))
+
+    def wide_depth(self):
+        return lib().vzo_bvh_wide_depth(self.h)
+
     def validate(self):
         return lib().vzo_bvh_validate(self.h, _p(self.spheres))
 
     def trace(self, rays):
+        """Closest hits over the 4-wide tree: (hits, wide node visits, sphere tests)."""
         rays = np.ascontiguousarray(rays, RAY_DT)
         hits = np.zeros(len(rays), HIT_DT)
         nv, ns = C.c_uint64(0), C.c_uint64(0)
         lib().vzo_trace(self.h, _p(self.spheres), _p(rays), len(rays), _p(hits), C.byref(nv),
                         C.byref(ns))
+        return hits, nv.value, ns.value
+
+    def trace_binary(self, rays):
+        """Closest hits over the binary tree: (hits, node visits, sphere tests)."""
+        rays = np.ascontiguousarray(rays, RAY_DT)
+        hits = np.zeros(len(rays), HIT_DT)
+        nv, ns = C.c_uint64(0), C.c_uint64(0)
+        lib().vzo_trace_binary(self.h, _p(self.spheres), _p(rays), len(rays), _p(hits), C.byref(nv),
+                               C.byref(ns))
         return hits, nv.value, ns.value
 
 
@@ -221,12 +244,15 @@ def trace_brute_f64(spheres, rays):
 
 
 def make_params(width, height, spp, max_depth, cam, subframe=1, row_start=0, row_step=1,
-                threads=0, accum_fp64=False):
+                threads=0, accum_fp64=False, binary_tree=False):
+    """binary_tree: count visits over the binary tree (what the GPU walks for
+    a scene of at most 513 spheres rendered alone); hits are the same."""
     p = RenderParams()
     p.width, p.height, p.spp, p.subframe, p.max_depth = width, height, spp, subframe, max_depth
     p.cam = cam
     p.row_start, p.row_step, p.threads = row_start, row_step, threads
     p.accum_fp64 = 1 if accum_fp64 else 0
+    p.binary_tree = 1 if binary_tree else 0
     return p
 
 

@@ -25,6 +25,10 @@ struct vzo_bvh {
     uint32_t* morton;     /* n, sorted */
     float* boxes;         /* (n-1)*6 internal-node boxes lo xyz, hi xyz */
     uint32_t depth;
+    /* the 4-wide tree (see collapse_wide) */
+    uint32_t nw;          /* wide nodes */
+    vzo_wnode* wnodes;
+    uint32_t wdepth;      /* wide levels from the root to the deepest leaf */
 };
 
 /* ---------------------------------------------------------------- RNG --- */
@@ -515,6 +519,124 @@ static void leaf_box(const vzo_sphere* s, float b[6])
     b[5] = s->cz + r;
 }
 
+/* Surface-area measure of a box: dx*dy + dy*dz + dz*dx, in that order. */
+static float box_area(const float b[6])
+{
+    float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+#define VZO_WIDE 4
+#define VZO_LEAF_REF 0x80000000u
+
+/* Children of the wide node rooted at binary internal node r: start with r's
+ * two children; while fewer than 4, open the internal child of largest box
+ * area (the first such slot on a tie) into its two children, in place, so
+ * the slots keep left-to-right leaf order.  Children are binary references:
+ * a leaf index | VZO_LEAF_REF, or an internal node index.  Returns the count. */
+static int wide_children(const uint32_t* gam, const uint32_t* flg, const float* boxes, uint32_t r,
+                         uint32_t out[VZO_WIDE])
+{
+    int cnt = 2;
+    for (int c = 0; c < 2; c++)
+        out[c] = ((flg[r] >> c) & 1u) ? ((gam[r] + (uint32_t)c) | VZO_LEAF_REF) : gam[r] + (uint32_t)c;
+    while (cnt < VZO_WIDE) {
+        int best = -1;
+        float best_a = 0.0f;
+        for (int k = 0; k < cnt; k++) {
+            if (out[k] & VZO_LEAF_REF) continue;
+            float a = box_area(&boxes[out[k] * 6]);
+            if (best < 0 || a > best_a) {
+                best = k;
+                best_a = a;
+            }
+        }
+        if (best < 0) break;
+        uint32_t b = out[best];
+        for (int k = cnt - 1; k > best; k--) out[k + 1] = out[k];
+        out[best] = ((flg[b] & 1u) ? (gam[b] | VZO_LEAF_REF) : gam[b]);
+        out[best + 1] = ((flg[b] & 2u) ? ((gam[b] + 1u) | VZO_LEAF_REF) : gam[b] + 1u);
+        cnt++;
+    }
+    return cnt;
+}
+
+/* The 4-wide tree: wide roots are binary node 0 and every internal child of a
+ * wide root; wide node indices number the roots in binary index order.  Each
+ * wide node is quantised like a binary node: relative to its own box (the
+ * binary box of its root) with a power-of-two scale per axis, planes rounded
+ * outward. */
+static void collapse_wide(vzo_bvh* b, const vzo_sphere* s, const uint32_t* gam, const uint32_t* flg)
+{
+    uint32_t ni = b->n - 1;
+    uint32_t* kids = (uint32_t*)calloc((size_t)ni * VZO_WIDE, sizeof(uint32_t));
+    uint8_t* cnt = (uint8_t*)calloc(ni, 1);
+    uint8_t* root = (uint8_t*)calloc(ni, 1);
+    uint32_t* level = (uint32_t*)calloc(ni, sizeof(uint32_t));
+    uint32_t* widx = (uint32_t*)calloc(ni, sizeof(uint32_t));
+    uint32_t* queue = (uint32_t*)malloc(sizeof(uint32_t) * ni);
+    uint32_t qh = 0, qt = 0, maxlev = 0;
+    root[0] = 1;
+    level[0] = 1;
+    queue[qt++] = 0;
+    while (qh < qt) {
+        uint32_t r = queue[qh++];
+        cnt[r] = (uint8_t)wide_children(gam, flg, b->boxes, r, &kids[(size_t)r * VZO_WIDE]);
+        if (level[r] > maxlev) maxlev = level[r];
+        for (int c = 0; c < cnt[r]; c++) {
+            uint32_t ch = kids[(size_t)r * VZO_WIDE + c];
+            if (ch & VZO_LEAF_REF) continue;
+            root[ch] = 1;
+            level[ch] = level[r] + 1;
+            queue[qt++] = ch;
+        }
+    }
+    uint32_t nw = 0;
+    for (uint32_t i = 0; i < ni; i++)
+        if (root[i]) widx[i] = nw++;
+    b->nw = nw;
+    b->wdepth = maxlev + 1;  /* wide node levels plus the leaf level */
+    b->wnodes = (vzo_wnode*)calloc(nw, sizeof(vzo_wnode));
+    for (uint32_t r = 0; r < ni; r++) {
+        if (!root[r]) continue;
+        vzo_wnode* W = &b->wnodes[widx[r]];
+        const float* nb = &b->boxes[r * 6];
+        uint32_t meta = 0;
+        float sc[3];
+        for (int k = 0; k < 3; k++) {
+            uint32_t e = pick_exp(nb[k], nb[3 + k]);
+            sc[k] = f_as(e << 23);
+            meta |= e << (8 * k);
+            W->o[k] = nb[k];
+        }
+        for (int c = 0; c < cnt[r]; c++) {
+            uint32_t ch = kids[(size_t)r * VZO_WIDE + c];
+            float cb[6];
+            if (ch & VZO_LEAF_REF) {
+                uint32_t leaf = ch & ~VZO_LEAF_REF;
+                leaf_box(&s[b->leaf_prim[leaf]], cb);
+                W->child[c] = leaf;
+                meta |= 1u << (24 + c);
+            } else {
+                memcpy(cb, &b->boxes[ch * 6], sizeof(cb));
+                W->child[c] = widx[ch];
+            }
+            meta |= 1u << (28 + c);
+            for (int k = 0; k < 3; k++) {
+                W->qlo[k] |= q_lo(cb[k], nb[k], sc[k]) << (8 * c);
+                W->qhi[k] |= q_hi(cb[3 + k], nb[k], sc[k]) << (8 * c);
+            }
+        }
+        W->meta = meta;
+    }
+    free(kids);
+    free(cnt);
+    free(root);
+    free(level);
+    free(widx);
+    free(queue);
+}
+
 vzo_bvh* vzo_bvh_build(const vzo_sphere* s, uint32_t n)
 {
     if (n == 0) return NULL;
@@ -638,6 +760,7 @@ vzo_bvh* vzo_bvh_build(const vzo_sphere* s, uint32_t n)
         }
         b->depth = maxd;
     }
+    collapse_wide(b, s, gam, flg);
     free(seen);
     free(parent_int);
     free(parent_leaf);
@@ -653,6 +776,7 @@ void vzo_bvh_free(vzo_bvh* b)
     free(b->leaf_prim);
     free(b->morton);
     free(b->boxes);
+    free(b->wnodes);
     free(b);
 }
 
@@ -663,6 +787,65 @@ const uint32_t* vzo_bvh_leaf_prims(const vzo_bvh* b) { return b->leaf_prim; }
 const uint32_t* vzo_bvh_morton(const vzo_bvh* b) { return b->morton; }
 const float* vzo_bvh_node_boxes(const vzo_bvh* b) { return b->boxes; }
 uint32_t vzo_bvh_depth(const vzo_bvh* b) { return b->depth; }
+uint32_t vzo_bvh_num_wide_nodes(const vzo_bvh* b) { return b->nw; }
+const vzo_wnode* vzo_bvh_wide_nodes(const vzo_bvh* b) { return b->wnodes; }
+uint32_t vzo_bvh_wide_depth(const vzo_bvh* b) { return b->n > 1 ? b->wdepth : 1; }
+
+/* Wide-tree invariants: node 0 is the root; every leaf and every other wide
+ * node is one slot's child exactly once; slots are filled from 0 (2 to 4 of
+ * them), empty slots are zero; each slot's dequantised box contains the
+ * exact box of its child (a leaf's sphere box, or the union of the child
+ * node's leaves); children cover consecutive leaf ranges in slot order.
+ * Returns 0 or a code from 20 up. */
+static int validate_wide(const vzo_bvh* b, const vzo_sphere* s, uint32_t w, uint32_t first, uint32_t* last,
+                         float box[6], uint8_t* ref_leaf, uint8_t* ref_w, int depth)
+{
+    if (depth > 256) return 20;
+    const vzo_wnode* W = &b->wnodes[w];
+    uint32_t valid = W->meta >> 28, leafm = (W->meta >> 24) & 0xFu;
+    if (valid != 0x3u && valid != 0x7u && valid != 0xFu) return 21;
+    if (leafm & ~valid) return 22;
+    uint32_t pos = first;
+    for (int k = 0; k < 3; k++) {
+        box[k] = 3.4e38f;
+        box[3 + k] = -3.4e38f;
+    }
+    for (int c = 0; c < VZO_WIDE; c++) {
+        if (!((valid >> c) & 1u)) {
+            if (W->child[c] != 0) return 23;
+            for (int k = 0; k < 3; k++)
+                if (((W->qlo[k] >> (8 * c)) & 0xFFu) || ((W->qhi[k] >> (8 * c)) & 0xFFu)) return 23;
+            continue;
+        }
+        float cb[6];
+        uint32_t ch = W->child[c];
+        if ((leafm >> c) & 1u) {
+            if (ch >= b->n || ch != pos) return 24;
+            ref_leaf[ch]++;
+            leaf_box(&s[b->leaf_prim[ch]], cb);
+            pos++;
+        } else {
+            if (ch == 0 || ch >= b->nw) return 25;
+            ref_w[ch]++;
+            uint32_t l;
+            int rc = validate_wide(b, s, ch, pos, &l, cb, ref_leaf, ref_w, depth + 1);
+            if (rc) return rc;
+            pos = l + 1;
+        }
+        for (int k = 0; k < 3; k++) {
+            float sc = vzo_scale_of(W->meta, k);
+            float dl = fmaf(sc, (float)((W->qlo[k] >> (8 * c)) & 0xFFu), W->o[k]);
+            float dh = fmaf(sc, (float)((W->qhi[k] >> (8 * c)) & 0xFFu), W->o[k]);
+            if (!(dl <= cb[k] && dh >= cb[3 + k])) return 26;
+            box[k] = fminf(box[k], cb[k]);
+            box[3 + k] = fmaxf(box[3 + k], cb[3 + k]);
+        }
+    }
+    for (int k = 0; k < 3; k++)
+        if (W->o[k] != box[k]) return 27;  /* the quantisation origin is the node box's lo corner */
+    *last = pos - 1;
+    return 0;
+}
 
 /* Structural invariants: each leaf and each non-root internal node is
  * referenced exactly once; parent boxes contain child boxes; dequantised
@@ -740,7 +923,39 @@ int vzo_bvh_validate(const vzo_bvh* b, const vzo_sphere* s)
     }
     free(ref_leaf);
     free(ref_int);
+    if (rc) return rc;
+    {
+        uint8_t* rl = (uint8_t*)calloc(n, 1);
+        uint8_t* rw = (uint8_t*)calloc(b->nw, 1);
+        uint32_t last = 0;
+        float box[6];
+        rc = b->nw == 0 ? 28 : validate_wide(b, s, 0, 0, &last, box, rl, rw, 0);
+        if (!rc && last != n - 1) rc = 29;
+        for (uint32_t i = 0; i < n && !rc; i++)
+            if (rl[i] != 1) rc = 30;
+        for (uint32_t i = 1; i < b->nw && !rc; i++)
+            if (rw[i] != 1) rc = 31;
+        free(rl);
+        free(rw);
+    }
     return rc;
+}
+
+void vzo_trace_binary(const vzo_bvh* b, const vzo_sphere* s, const vzo_ray* rays, uint32_t n,
+                      vzo_hit* hits, uint64_t* node_visits, uint64_t* sphere_tests)
+{
+    uint64_t nv = 0, ns = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const vzo_ray* r = &rays[i];
+        float t;
+        uint32_t p;
+        closest_bin_f32(b, s, mk_f32(r->ox, r->oy, r->oz), mk_f32(r->dx, r->dy, r->dz), r->tmin, r->tmax,
+                        &t, &p, &nv, &ns);
+        hits[i].t = (p == VZO_MISS) ? r->tmax : t;
+        hits[i].prim = p;
+    }
+    if (node_visits) *node_visits = nv;
+    if (sphere_tests) *sphere_tests = ns;
 }
 
 void vzo_trace(const vzo_bvh* b, const vzo_sphere* s, const vzo_ray* rays, uint32_t n,
@@ -900,8 +1115,8 @@ static void* worker(void* arg)
     job_t* j = w->j;
     const vzo_render_params* p = j->p;
     vzo_stats* st = &j->st[w->id];
-    pctx_f32 c32 = {j->b, j->s, j->m, j->nsph, p->max_depth, NULL, 0, 0};
-    pctx_f64 c64 = {j->b, j->s, j->m, j->nsph, p->max_depth, NULL, 0, 0};
+    pctx_f32 c32 = {j->b, j->s, j->m, j->nsph, p->max_depth, p->binary_tree, NULL, 0, 0};
+    pctx_f64 c64 = {j->b, j->s, j->m, j->nsph, p->max_depth, p->binary_tree, NULL, 0, 0};
     uint32_t ntiles = j->ntiles_x * j->ntiles_y;
     for (;;) {
         uint32_t t = __atomic_fetch_add(&j->next, 1u, __ATOMIC_RELAXED);
@@ -1017,7 +1232,7 @@ void vzo_make_rayset(const vzo_bvh* b, const vzo_sphere* s, const vzo_material* 
                      const vzo_render_params* p, uint32_t n_int, uint32_t n_iso, uint32_t iso_seed,
                      vzo_ray* out)
 {
-    pctx_f32 c = {b, s, m, nsph, p->max_depth, out, 0, n_int};
+    pctx_f32 c = {b, s, m, nsph, p->max_depth, p->binary_tree, out, 0, n_int};
     vzo_stats st;
     memset(&st, 0, sizeof(st));
     const uint32_t step = p->row_step ? p->row_step : 1u;

@@ -48,6 +48,12 @@ typedef struct { uint32_t type; float albedo[3]; float fuzz; float ir; } vzo_mat
 typedef struct { float ox, oy, oz, tmin, dx, dy, dz, tmax; } vzo_ray;    /* 32 B */
 typedef struct { float t; uint32_t prim; } vzo_hit;                      /* 8 B, prim=~0 on miss */
 typedef struct { float o[3]; uint32_t meta; uint32_t q[3]; uint32_t child; } vzo_node; /* 32 B */
+/* 4-wide node (64 B): the node box's lo corner, per-axis scale exponents
+ * (meta bytes 0-2), slot leaf flags (meta bits 24-27) and valid flags (bits
+ * 28-31); per axis the four slots' quantised lo planes (byte c = slot c) and
+ * hi planes; per slot the child: a leaf index (Morton order) or a wide node
+ * index. */
+typedef struct { float o[3]; uint32_t meta; uint32_t qlo[3]; uint32_t qhi[3]; uint32_t child[4]; uint32_t pad[2]; } vzo_wnode;
 typedef struct { float origin[3], u[3], v[3], w[3]; float lens_radius; } vzo_camera;
 
 enum { VZO_LAMBERTIAN = 0, VZO_METAL = 1, VZO_DIELECTRIC = 2 };
@@ -87,12 +93,21 @@ const uint32_t* vzo_bvh_leaf_prims(const vzo_bvh* b);    /* leaf -> primitive id
 const uint32_t* vzo_bvh_morton(const vzo_bvh* b);        /* sorted Morton codes */
 const float* vzo_bvh_node_boxes(const vzo_bvh* b);       /* internal boxes, 6 floats each */
 uint32_t vzo_bvh_depth(const vzo_bvh* b);
-/* Returns 0 when every structural invariant holds, else a nonzero code. */
+/* The 4-wide tree collapsed from the binary one (root = wide node 0). */
+uint32_t vzo_bvh_num_wide_nodes(const vzo_bvh* b);
+const vzo_wnode* vzo_bvh_wide_nodes(const vzo_bvh* b);
+uint32_t vzo_bvh_wide_depth(const vzo_bvh* b);
+/* Returns 0 when every structural invariant holds (binary and wide trees),
+ * else a nonzero code. */
 int vzo_bvh_validate(const vzo_bvh* b, const vzo_sphere* s);
 
-/* Closest hit over a ray array (tmin/tmax from each ray). */
+/* Closest hit over a ray array (tmin/tmax from each ray), traversing the
+ * 4-wide tree; node_visits counts wide nodes.  vzo_trace_binary traverses
+ * the binary tree instead (same hits, its own counts). */
 void vzo_trace(const vzo_bvh* b, const vzo_sphere* s, const vzo_ray* rays, uint32_t n,
                vzo_hit* hits, uint64_t* node_visits, uint64_t* sphere_tests);
+void vzo_trace_binary(const vzo_bvh* b, const vzo_sphere* s, const vzo_ray* rays, uint32_t n,
+                      vzo_hit* hits, uint64_t* node_visits, uint64_t* sphere_tests);
 void vzo_trace_brute(const vzo_sphere* s, uint32_t nsph, const vzo_ray* rays, uint32_t n,
                      vzo_hit* hits);
 void vzo_trace_brute_f64(const vzo_sphere* s, uint32_t nsph, const vzo_ray* rays, uint32_t n,
@@ -109,8 +124,11 @@ typedef struct {
     uint32_t width, height, spp, subframe, max_depth;
     vzo_camera cam;
     uint32_t row_start, row_step;   /* render rows y = row_start + k*row_step */
-    uint32_t threads;               /* 0 = all hardware threads */
+    uint32_t threads;               /* 0 = the CPUs this process may run on */
     uint32_t accum_fp64;            /* accum buffer is double4 instead of float4 */
+    uint32_t binary_tree;           /* traverse the binary tree instead of the 4-wide one (same
+                                       hits; the visit counts of a small scene rendered alone on
+                                       the GPU, whose binary tree is staged in LDS) */
 } vzo_render_params;
 
 typedef struct {

@@ -26,6 +26,7 @@ int main(void)
     F(vzRequestDesc, scene_key); F(vzRequestDesc, width); F(vzRequestDesc, flags);
     F(vzRequestDesc, subframe_index); F(vzRequestDesc, lookfrom); F(vzRequestDesc, lookat);
     F(vzRequestDesc, vfov_deg); F(vzRequestDesc, focus_dist); F(vzRequestDesc, arrival_ms);
+    F(vzAccelInfo, build_ms); F(vzAccelInfo, num_wide_nodes); F(vzAccelInfo, wide_depth);
     printf("\"end\": 0}\n");
     return 0;
 }

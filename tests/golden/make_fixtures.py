@@ -83,12 +83,16 @@ def main():
     bvh = O.BVH(s)
     assert bvh.validate() == 0
     meta = {"n": int(len(s)), "nodes_sha256": sha(bvh.nodes()), "leaf_sha256": sha(bvh.leaf_prims()),
-            "morton_sha256": sha(bvh.morton()), "depth": int(bvh.depth())}
+            "morton_sha256": sha(bvh.morton()), "depth": int(bvh.depth()),
+            "wide_nodes_sha256": sha(bvh.wide_nodes()), "wide_nodes": int(len(bvh.wide_nodes())),
+            "wide_depth": int(bvh.wide_depth())}
 
     cam = book_camera(C1["width"], C1["height"])
     p = O.make_params(C1["width"], C1["height"], C1["spp"], C1["max_depth"], cam)
     rays = O.make_rayset(s, m, p, N_INT, N_ISO, ISO_SEED, bvh=bvh)
     hits, nv, ns = bvh.trace(rays)
+    hits_b, nv_b, ns_b = bvh.trace_binary(rays)
+    assert hits_b.tobytes() == hits.tobytes()
     sub = np.arange(0, len(rays), 256)
     brute = O.trace_brute(s, rays[sub])
     assert brute.tobytes() == hits[sub].tobytes()
@@ -96,7 +100,8 @@ def main():
     np.save(os.path.join(OUT, "rayset_4096_hits.npy"), hits[sub])
     meta["rayset"] = {"n_int": N_INT, "n_iso": N_ISO, "iso_seed": ISO_SEED, "config": C1,
                       "rays_sha256": sha(rays), "hits_sha256": sha(hits), "node_visits": int(nv),
-                      "sphere_tests": int(ns), "hit_count": int((hits["prim"] != O.MISS).sum())}
+                      "sphere_tests": int(ns), "hit_count": int((hits["prim"] != O.MISS).sum()),
+                      "binary_node_visits": int(nv_b), "binary_sphere_tests": int(ns_b)}
 
     cam0 = book_camera(400, 225)
     p0 = O.make_params(400, 225, 16, 50, cam0)

@@ -39,6 +39,7 @@ static int run_scene(vzo_sphere* s, vzo_material* m, uint32_t n, int fp64_accum)
     p.row_step = 1;
     p.threads = 2;
     p.accum_fp64 = (uint32_t)fp64_accum;
+    p.binary_tree = (uint32_t)fp64_accum;
     const size_t npx = (size_t)p.width * p.height;
     void* acc = xalloc(npx * 4 * (fp64_accum ? sizeof(double) : sizeof(float)));
     uint8_t* img = (uint8_t*)xalloc(npx * 4);

@@ -34,6 +34,10 @@ def compare_bvh(scene, spheres):
     assert np.array_equal(leaf, ob.leaf_prims())
     assert nodes.tobytes() == ob.nodes().tobytes()
     assert scene.info().depth == ob.depth()
+    # the 4-wide tree every traversal walks, collapsed on the GPU
+    wide = scene.download_wide()
+    assert wide.tobytes() == ob.wide_nodes().tobytes()
+    assert scene.info().num_wide_nodes == len(wide) and scene.info().wide_depth == ob.wide_depth()
     return ob
 
 
@@ -45,6 +49,7 @@ def test_lbvh_book_bit_exact(ctx, book):
     assert sha(nodes) == FIX["nodes_sha256"]
     assert sha(leaf) == FIX["leaf_sha256"]
     assert sha(morton) == FIX["morton_sha256"]
+    assert sha(sc.download_wide()) == FIX["wide_nodes_sha256"]
     compare_bvh(sc, s)
 
 
@@ -189,14 +194,30 @@ def test_render_box_scene(ctx):
 
 
 def test_visit_counts_match_oracle(ctx, book):
-    say("case: visit counters")
+    """The book scene rendered alone walks its binary tree from LDS; rendered
+    in a batch next to another scene it walks the 4-wide tree.  Both visit
+    counts match the oracle traversing the same tree."""
+    say("case: visit counters (binary tree alone, 4-wide tree in a batch)")
     s, m = book
     cam = book_lookat(80, 45)
     sc = api.Scene(ctx, s, m)
     tgt = GpuTarget(80, 45)
     st = sc.launch(tgt.bind(gpu_params(80, 45, 4, 50, cam, flags=api.COUNT_VISITS)))
-    _, _, ost = O.render(s, m, oracle_params(80, 45, 4, 50, cam), bvh=O.BVH(s))
+    ob = O.BVH(s)
+    _, _, ost = O.render(s, m, oracle_params(80, 45, 4, 50, cam, binary_tree=True), bvh=ob)
     assert (st.segments, st.node_visits, st.sphere_tests) == (ost.segments, ost.node_visits, ost.sphere_tests)
+    bs, bm = api.box_scene(3000, 12)
+    sc2 = api.Scene(ctx, bs, bm)
+    tgt2 = GpuTarget(40, 30)
+    p1 = tgt.bind(gpu_params(80, 45, 4, 50, cam, flags=api.COUNT_VISITS))
+    p2 = tgt2.bind(gpu_params(40, 30, 2, 16, box_lookat(40, 30), flags=api.COUNT_VISITS))
+    bst = api.batch_launch(ctx, [(sc, p1), (sc2, p2)])
+    _, _, w1 = O.render(s, m, oracle_params(80, 45, 4, 50, cam), bvh=ob)
+    _, _, w2 = O.render(bs, bm, oracle_params(40, 30, 2, 16, box_lookat(40, 30)), bvh=O.BVH(bs))
+    assert w1.node_visits < ost.node_visits  # the 4-wide tree visits fewer nodes
+    assert (bst.segments, bst.node_visits, bst.sphere_tests) == (
+        w1.segments + w2.segments, w1.node_visits + w2.node_visits, w1.sphere_tests + w2.sphere_tests)
+    sc2.close()
 
 
 def test_refit_after_motion(ctx):

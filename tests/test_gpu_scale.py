@@ -55,6 +55,8 @@ def test_c2_lbvh_bit_exact(c2):
     assert np.array_equal(leaf, ob.leaf_prims())
     assert nodes.tobytes() == ob.nodes().tobytes()
     assert sc.info().depth == ob.depth()
+    assert sc.download_wide().tobytes() == ob.wide_nodes().tobytes()
+    assert sc.info().wide_depth == ob.wide_depth()
 
 
 def test_c2_rayset_bit_exact(c2):

@@ -22,11 +22,11 @@ def box_lookat(w, h):
                 aspect=float(np.float32(w) / np.float32(h)))
 
 
-def oracle_params(w, h, spp, depth, cam, subframe=1, accum_fp64=False, row_step=1):
+def oracle_params(w, h, spp, depth, cam, subframe=1, accum_fp64=False, row_step=1, binary_tree=False):
     c = O.camera_lookat(cam["lookfrom"], cam["lookat"], cam["vfov"], cam["aspect"],
                         cam["aperture"], cam["focus"])
     return O.make_params(w, h, spp, depth, c, subframe=subframe, accum_fp64=accum_fp64,
-                         row_step=row_step)
+                         row_step=row_step, binary_tree=binary_tree)
 
 
 def gpu_params(w, h, spp, depth, cam, subframe=1, flags=0):

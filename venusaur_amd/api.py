@@ -31,6 +31,8 @@ RAY_DT = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"),
                    ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
 HIT_DT = np.dtype([("t", "<f4"), ("prim", "<u4")])
 NODE_DT = np.dtype([("o", "<f4", 3), ("meta", "<u4"), ("q", "<u4", 3), ("child", "<u4")])
+WNODE_DT = np.dtype([("o", "<f4", 3), ("meta", "<u4"), ("qlo", "<u4", 3), ("qhi", "<u4", 3),
+                     ("child", "<u4", 4), ("pad", "<u4", 2)])
 
 
 class VzError(RuntimeError):
@@ -58,7 +60,8 @@ class Stats(C.Structure):
 
 class AccelInfo(C.Structure):
     _fields_ = [("num_prims", C.c_uint32), ("num_nodes", C.c_uint32), ("depth", C.c_uint32),
-                ("node_bytes", C.c_uint32), ("device_bytes", C.c_uint64), ("build_ms", C.c_double)]
+                ("node_bytes", C.c_uint32), ("device_bytes", C.c_uint64), ("build_ms", C.c_double),
+                ("num_wide_nodes", C.c_uint32), ("wide_depth", C.c_uint32)]
 
 
 class BatchItem(C.Structure):
@@ -110,6 +113,7 @@ def _declare(L):
         "vzAccelRefit": (st, [vp, vp]),
         "vzAccelInfoGet": (st, [vp, P(AccelInfo)]),
         "vzAccelDownload": (st, [vp, vp, vp, vp, vp]),
+        "vzAccelDownloadWide": (st, [vp, vp, vp]),
         "vzTraceClosest": (st, [vp, vp, u32, vp, vp, P(Stats)]),
         "vzLaunch": (st, [vp, P(LaunchParams), vp, P(Stats)]),
         "vzBatchLaunch": (st, [vp, P(BatchItem), u32, P(Tuning), vp, P(Stats)]),
@@ -380,6 +384,13 @@ class Scene:
         check(lib().vzAccelDownload(self.h, _p(nodes) if n > 1 else None, _p(leaf), _p(morton),
                                     stream))
         return nodes, leaf, morton
+
+    def download_wide(self, stream=None):
+        """The 4-wide tree every traversal walks (vzAccelDownloadWide)."""
+        nw = self.info().num_wide_nodes
+        w = np.zeros(nw, WNODE_DT)
+        check(lib().vzAccelDownloadWide(self.h, _p(w) if nw else C.c_void_p(1), stream))
+        return w
 
     def trace(self, rays_dev_ptr, n, hits_dev_ptr, stream=None, stats=False):
         st = Stats()

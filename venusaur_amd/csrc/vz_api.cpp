@@ -115,11 +115,8 @@ vzStatus vzCreate(int device, vzContext* out)
     VZ_HIP_TRY(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return set_error(VZ_ERROR_NO_DEVICE, std::string("libvz is built for gfx950, device is ") + prop.gcnArchName);
-    unsigned long long* scratch = nullptr;
-    VZ_TRY(trace_scratch_address(&scratch));  // this device's copy, looked up once with the device current
     vzContext c = new vzContext_st();
     c->device = device;
-    c->trace_scratch = scratch;
     c->num_cus = (uint32_t)prop.multiProcessorCount;
     c->hbm_bytes = (uint64_t)prop.totalGlobalMem;
     // the marketing name can be empty (no entry in the driver's device-id
@@ -296,6 +293,18 @@ vzStatus vzAccelInfoGet(vzScene scene, vzAccelInfo* out)
     out->node_bytes = 32;
     out->device_bytes = scene->device_bytes;
     out->build_ms = scene->build_ms;
+    out->num_wide_nodes = scene->nw;
+    out->wide_depth = scene->wdepth;
+    return VZ_SUCCESS;
+}
+
+vzStatus vzAccelDownloadWide(vzScene scene, void* wide_nodes, vzStream stream)
+{
+    if (!scene || !wide_nodes) return set_error(VZ_ERROR_INVALID_VALUE, "NULL argument");
+    hipStream_t st = as_stream(stream);
+    if (scene->nw)
+        VZ_HIP_TRY(hipMemcpyAsync(wide_nodes, scene->d_wnodes, (size_t)scene->nw * 64, hipMemcpyDeviceToHost, st));
+    VZ_HIP_TRY(hipStreamSynchronize(st));
     return VZ_SUCCESS;
 }
 

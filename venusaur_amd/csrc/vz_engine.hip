@@ -7,16 +7,19 @@
 #include <cstring>
 #include <string>
 
+#include <type_traits>
+
 #include "vz_engine.h"
 #include "vz_traverse.h"
+#include "vz_traverse_bin.h"
 
 namespace vz {
 namespace {
 
 constexpr int kStepBlk = 512;   // step-kernel workgroup (8 waves)
-constexpr int kStackS = 8;      // LDS stack slots per lane
+constexpr int kStackS = kWideStackS;  // LDS stack slots per lane
 constexpr int kTbl = 128;       // LDS (request, pass) tally table entries
-constexpr uint32_t kLdsNodes = 512;  // scenes up to this many nodes are staged in LDS
+constexpr uint32_t kLdsNodes = 512;  // a scene of at most this many binary nodes is staged in LDS (16 KB)
 // A wave goes back for new rays once fewer than this many of its lanes still
 // trace (1 = only when all are done, 65 = after every node step).  Defaults per
 // node source, chosen by sweeping the 1080p book scene (LDS nodes) and
@@ -68,6 +71,23 @@ __device__ __forceinline__ void terminate(const ReqDesc& D, uint32_t slot, uint3
     pb[2u * D.npix + pixel] = L.z;
 }
 
+// One node visit of either traversal: the binary tree staged in LDS, or the
+// 4-wide tree from memory.
+template <int S, bool COUNT>
+__device__ __forceinline__ bool step_node(BinTraversal<S, COUNT>& tr, const SceneView& sc, const uint4* s_bvh,
+                                          uint32_t* lds_node, float* lds_t, BinOverflow& ov, uint32_t& nv,
+                                          uint32_t& ns)
+{
+    return tr.step(sc, LdsBinNodes{s_bvh}, lds_node, lds_t, kStepBlk, ov, nv, ns);
+}
+template <int S, bool COUNT>
+__device__ __forceinline__ bool step_node(Traversal<S, COUNT>& tr, const SceneView& sc, const uint4*,
+                                          uint32_t* lds_node, float* lds_t, Overflow& ov, uint32_t& nv,
+                                          uint32_t& ns)
+{
+    return tr.step(sc, GlobalNodes{sc.nodes}, lds_node, lds_t, kStepBlk, ov, nv, ns);
+}
+
 // ---- step: trace + miss/shade stages + compaction, one slice per workgroup ----
 // Persistent traversal with per-lane refill (Aila & Laine, "Understanding the
 // Efficiency of Ray Traversal on GPUs", HPG 2009): a lane whose ray has
@@ -98,7 +118,7 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
     const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
     if (LDSN) {  // one small scene: stage its node array in LDS
         const uint32_t nn = 2u * (scenes[0].n - 1u);
-        const uint4* src = scenes[0].nodes;
+        const uint4* src = scenes[0].bnodes;
         for (uint32_t i = tid; i < nn; i += kStepBlk) s_bvh[i] = src[i];
     }
     for (uint32_t i = tid; i < (uint32_t)kTbl; i += kStepBlk) {
@@ -117,8 +137,10 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
     uint32_t nv = 0, ns = 0, nh = 0, seg = 0;
     SceneView sc;
     if (SINGLE) sc = scenes[0];
-    Traversal<kStackS, COUNT> tr;
-    Overflow ov;
+    // a small scene rendered alone walks its binary tree from LDS; every
+    // other launch walks the 4-wide tree from memory
+    typename std::conditional<LDSN, BinTraversal<kStackS, COUNT>, Traversal<kStackS, COUNT>>::type tr;
+    typename std::conditional<LDSN, BinOverflow, Overflow>::type ov;
     bool active = false, tdone = false;
     size_t qi = 0;
     uint32_t meta = 0;
@@ -154,19 +176,11 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
         // of many lanes together, was measured slower at every threshold.)
         if (refill <= 1u) {  // the whole wave finishes: a plain divergent loop
             while (active && !tdone) {
-                if (LDSN)
-                    tdone = tr.step(sc, LdsNodes{s_bvh}, s_node + tid, s_t + tid, kStepBlk, ov, nv, ns);
-                else
-                    tdone = tr.step(sc, GlobalNodes{sc.nodes}, s_node + tid, s_t + tid, kStepBlk, ov, nv, ns);
+                tdone = step_node(tr, sc, s_bvh, s_node + tid, s_t + tid, ov, nv, ns);
             }
         } else {
             for (;;) {
-                if (active && !tdone) {
-                    if (LDSN)
-                        tdone = tr.step(sc, LdsNodes{s_bvh}, s_node + tid, s_t + tid, kStepBlk, ov, nv, ns);
-                    else
-                        tdone = tr.step(sc, GlobalNodes{sc.nodes}, s_node + tid, s_t + tid, kStepBlk, ov, nv, ns);
-                }
+                if (active && !tdone) tdone = step_node(tr, sc, s_bvh, s_node + tid, s_t + tid, ov, nv, ns);
                 const uint64_t busy = __ballot(active && !tdone);
                 if (busy == 0ull) break;
                 if (!exhausted && (uint32_t)__popcll(busy) < refill) break;

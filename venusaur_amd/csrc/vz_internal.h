@@ -18,6 +18,13 @@ namespace vz {
 // the deepest leaf).  Karras trees over 30-bit codes with index tie-breaks
 // stay well below this for any scene that fits in memory.
 constexpr uint32_t kMaxDepth = 64;
+// Traversal stack of the 4-wide tree: up to 3 entries per wide level above
+// the node being visited, kWideStackS of them per lane in LDS and the oldest
+// kWideOverflow in private memory.  Scenes whose wide depth needs more are
+// refused at build time.
+constexpr uint32_t kWideStackS = 8;
+constexpr uint32_t kWideOverflow = 96;
+constexpr uint32_t kMaxWideDepth = (kWideOverflow + kWideStackS) / 3 + 1;
 constexpr uint32_t kMaxRequests = 1024;  // device request-table size
 
 // Records a message for vzGetLastError and returns the status.
@@ -43,12 +50,13 @@ vzStatus check_spheres(const vzSphere* s, uint32_t n);
     } while (0)
 
 struct SceneView {           // what kernels need to traverse one scene
-    const uint4* nodes;      // 2 uint4 per node
+    const uint4* nodes;      // 4-wide nodes, 4 uint4 (64 B) each, root = 0
+    const uint4* bnodes;     // binary nodes, 2 uint4 (32 B) each, root = 0 (staged in LDS for small scenes)
     const float4* spheres;   // leaf order {c, r}
     const uint32_t* leaf_info;  // leaf -> prim id | material type << 30
     const float4* mats;      // leaf order {albedo rgb, fuzz | ior}
-    uint32_t n;
-    uint32_t pad;
+    uint32_t n;              // primitives (leaves)
+    uint32_t nw;             // 4-wide nodes (0 for a single sphere)
 };
 
 struct Engine;
@@ -129,18 +137,28 @@ struct vzScene_st {
     uint32_t* d_height = nullptr;
     uint32_t* d_frontier = nullptr;   // refit: finished children of workgroup-crossing nodes
     uint32_t* d_fcount = nullptr;
+    // the 4-wide tree collapsed from the binary one (traversed by every kernel)
+    uint4* d_wnodes = nullptr;        // 4 uint4 per wide node, at most n-1 of them
+    uint32_t* d_wkids = nullptr;      // 4 binary child references per wide root
+    uint32_t* d_wflag = nullptr;      // per binary node: wide-root bit and child count
+    uint32_t* d_widx = nullptr;       // per binary node: its wide node index (roots only)
+    uint32_t* d_wfront[2] = {nullptr, nullptr};  // level frontiers
+    uint32_t* d_wcnt = nullptr;       // per level frontier counts, then nw and the wide depth
+    uint32_t nw = 0;                  // wide nodes
+    uint32_t wdepth = 0;              // wide levels from the root to the deepest leaf
     uint32_t depth = 0;
     uint64_t device_bytes = 0;
     double build_ms = 0.0;
     vz::SceneView view() const
     {
         vz::SceneView v;
-        v.nodes = d_nodes;
+        v.nodes = d_wnodes;
+        v.bnodes = d_nodes;
         v.spheres = d_sph;
         v.leaf_info = d_leaf_info;
         v.mats = d_mat;
         v.n = n;
-        v.pad = 0;
+        v.nw = nw;
         return v;
     }
 };

@@ -506,8 +506,10 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit_local(const float4* __r
             }
             pack_node(p, cb, g, flags, nb, nodes);
             nh = h + 1u;
-            for (int k = 0; k < 6; k++)
+            for (int k = 0; k < 6; k++) {
                 __hip_atomic_store(&s_box[k][slot], nb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                boxes[(size_t)p * 6 + k] = nb[k];  // every node's exact box: the 4-wide collapse reads them
+            }
             __hip_atomic_store(&s_h[slot], nh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             done = p;
             if (p == 0u) break;  // the whole tree was local
@@ -563,6 +565,189 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit_top(const float4* __res
             p = parent_int[p] & ~kLocalBit;
         }
     }
+}
+
+// ---- 7. the 4-wide tree ---------------------------------------------------------
+// Collapsed top down from the binary tree, level by level: a wide root's
+// children start as its binary node's two children; while there are fewer
+// than 4, the internal child of largest box area (the first on a tie) is
+// opened into its two children in place, so slots keep left-to-right leaf
+// order.  Every internal child becomes a wide root of the next level.  Wide
+// indices number the roots in binary index order (an exclusive scan of the
+// root flags), so the tree does not depend on the order threads ran in.
+// Each wide node (64 bytes) is quantised relative to its own box like the
+// binary nodes: lo corner, per-axis power-of-two scale, planes rounded
+// outward; meta holds the scale exponents, the slots' leaf flags (bits
+// 24-27) and valid flags (28-31).
+constexpr uint32_t kWideRoot = 0x80000000u;  // wide-root flag of a binary node
+constexpr uint32_t kLeafRef = 0x80000000u;   // a wide child that is a leaf
+constexpr uint32_t kScanItems = 4;            // root flags per thread in the index scan
+constexpr uint32_t kScanTile = kBlk * kScanItems;
+
+__device__ __forceinline__ float box_area(const float* b)
+{
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ __forceinline__ uint32_t child_ref(uint32_t cw, int c)
+{
+    const uint32_t g = (cw & 0x3FFFFFFFu) + (uint32_t)c;
+    return ((cw >> (30 + c)) & 1u) ? (g | kLeafRef) : g;
+}
+
+__global__ void __launch_bounds__(kBlk) k_wide_init(uint32_t* __restrict__ wflag, uint32_t* __restrict__ front,
+                                                    uint32_t* __restrict__ cnt, uint32_t ncnt)
+{
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < ncnt; i += kBlk) cnt[i] = i == 0u ? 1u : 0u;
+    if (t == 0) {
+        wflag[0] = kWideRoot;
+        front[0] = 0u;
+    }
+}
+
+// One level: every wide root of the frontier picks its children; internal
+// children are the next level's frontier.  cnt[level + 1] starts at zero.
+__global__ void __launch_bounds__(kBlk) k_wide_level(const uint32_t* __restrict__ child,
+                                                     const float* __restrict__ boxes,
+                                                     const uint32_t* __restrict__ front_in,
+                                                     uint32_t* __restrict__ front_out, uint32_t* __restrict__ cnt,
+                                                     uint32_t level, uint32_t* __restrict__ wkids,
+                                                     uint32_t* __restrict__ wflag, uint32_t* __restrict__ wdepth)
+{
+    const uint32_t count = cnt[level];
+    for (uint32_t i = blockIdx.x * kBlk + threadIdx.x; i < count; i += gridDim.x * kBlk) {
+        const uint32_t r = front_in[i];
+        const uint32_t cw = child[r];
+        uint32_t ks[4] = {child_ref(cw, 0), child_ref(cw, 1), 0u, 0u};
+        uint32_t n_k = 2;
+        while (n_k < 4u) {
+            int best = -1;
+            float ba = 0.0f;
+            for (uint32_t k = 0; k < n_k; k++) {
+                if (ks[k] & kLeafRef) continue;
+                const float a = box_area(boxes + (size_t)ks[k] * 6);
+                if (best < 0 || a > ba) {
+                    best = (int)k;
+                    ba = a;
+                }
+            }
+            if (best < 0) break;
+            const uint32_t ow = child[ks[best]];
+            for (int k = (int)n_k - 1; k > best; k--) ks[k + 1] = ks[k];
+            ks[best] = child_ref(ow, 0);
+            ks[best + 1] = child_ref(ow, 1);
+            n_k++;
+        }
+        reinterpret_cast<uint4*>(wkids)[r] = make_uint4(ks[0], ks[1], ks[2], ks[3]);
+        wflag[r] = kWideRoot | n_k;
+        for (uint32_t k = 0; k < n_k; k++) {
+            const uint32_t c = ks[k];
+            if (c & kLeafRef) continue;
+            wflag[c] = kWideRoot;
+            front_out[atomicAdd(&cnt[level + 1], 1u)] = c;
+        }
+        atomicMax(wdepth, level + 2u);  // wide levels 1..level+1, plus the leaf level
+    }
+}
+
+// Wide indices: an exclusive scan of the root flags over the binary nodes,
+// in tiles of kScanTile (tile totals, their scan, then each tile's own).
+__global__ void __launch_bounds__(kBlk) k_wscan_tiles(const uint32_t* __restrict__ wflag, uint32_t ni,
+                                                      uint32_t* __restrict__ tsum)
+{
+    __shared__ uint32_t s[kBlk];
+    uint32_t c = 0;
+    const uint32_t base = blockIdx.x * kScanTile;
+    for (uint32_t r = 0; r < kScanItems; r++) {
+        const uint32_t i = base + r * kBlk + threadIdx.x;
+        if (i < ni && (wflag[i] & kWideRoot)) c++;
+    }
+    const uint32_t e = block_scan_excl(c, s);
+    if (threadIdx.x == kBlk - 1) tsum[blockIdx.x] = e + c;
+}
+
+__global__ void __launch_bounds__(kBlk) k_wscan_top(uint32_t* __restrict__ tsum, uint32_t nt,
+                                                    uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t part[kBlk];
+    const uint32_t chunk = (nt + kBlk - 1) / kBlk;
+    const uint32_t b = threadIdx.x * chunk, e = min(b + chunk, nt);
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; i++) sum += tsum[i];
+    const uint32_t excl = block_scan_excl(sum, part);
+    uint32_t run = excl;
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t v = tsum[i];
+        tsum[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == kBlk - 1) *total = excl + sum;
+}
+
+__global__ void __launch_bounds__(kBlk) k_wscan_apply(const uint32_t* __restrict__ wflag, uint32_t ni,
+                                                      const uint32_t* __restrict__ tsum,
+                                                      uint32_t* __restrict__ widx)
+{
+    __shared__ uint32_t s[kBlk];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;  // items contiguous per thread
+    uint32_t f[kScanItems], c = 0;
+    for (uint32_t r = 0; r < kScanItems; r++) {
+        const uint32_t i = base + r;
+        f[r] = (i < ni && (wflag[i] & kWideRoot)) ? 1u : 0u;
+        c += f[r];
+    }
+    uint32_t run = tsum[blockIdx.x] + block_scan_excl(c, s);
+    for (uint32_t r = 0; r < kScanItems; r++) {
+        const uint32_t i = base + r;
+        if (f[r]) widx[i] = run;
+        run += f[r];
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_wide_pack(const float4* __restrict__ sph, const uint32_t* __restrict__ wkids,
+                                                    const uint32_t* __restrict__ wflag,
+                                                    const uint32_t* __restrict__ widx,
+                                                    const float* __restrict__ boxes, uint32_t ni,
+                                                    uint4* __restrict__ wnodes)
+{
+    const uint32_t r = blockIdx.x * kBlk + threadIdx.x;
+    if (r >= ni) return;
+    const uint32_t fl = wflag[r];
+    if (!(fl & kWideRoot)) return;
+    const uint32_t cnt = fl & 7u;
+    const float* nb = boxes + (size_t)r * 6;
+    uint32_t meta = 0, qlo[3] = {0u, 0u, 0u}, qhi[3] = {0u, 0u, 0u}, ch[4] = {0u, 0u, 0u, 0u};
+    float sc[3];
+    for (int k = 0; k < 3; k++) {
+        const uint32_t e = pick_exp(nb[k], nb[3 + k]);
+        sc[k] = __uint_as_float(e << 23);
+        meta |= e << (8 * k);
+    }
+    for (uint32_t c = 0; c < cnt; c++) {
+        const uint32_t ref = wkids[(size_t)r * 4 + c];
+        float cb[6];
+        if (ref & kLeafRef) {
+            const uint32_t leaf = ref & ~kLeafRef;
+            leaf_box(sph[leaf], cb);
+            ch[c] = leaf;
+            meta |= 1u << (24 + c);
+        } else {
+            for (int k = 0; k < 6; k++) cb[k] = boxes[(size_t)ref * 6 + k];
+            ch[c] = widx[ref];
+        }
+        meta |= 1u << (28 + c);
+        for (int k = 0; k < 3; k++) {
+            qlo[k] |= quant_lo(cb[k], nb[k], sc[k]) << (8 * c);
+            qhi[k] |= quant_hi(cb[3 + k], nb[k], sc[k]) << (8 * c);
+        }
+    }
+    uint4* w = wnodes + (size_t)widx[r] * 4;
+    w[0] = make_uint4(__float_as_uint(nb[0]), __float_as_uint(nb[1]), __float_as_uint(nb[2]), meta);
+    w[1] = make_uint4(qlo[0], qlo[1], qlo[2], qhi[0]);
+    w[2] = make_uint4(qhi[1], qhi[2], ch[0], ch[1]);
+    w[3] = make_uint4(ch[2], ch[3], 0u, 0u);
 }
 
 __global__ void k_zero(uint32_t* a, uint32_t n)
@@ -627,6 +812,13 @@ vzStatus lbvh_alloc(vzScene s)
     VZ_TRY(al((void**)&s->d_height, sizeof(uint32_t) * ni));
     VZ_TRY(al((void**)&s->d_frontier, sizeof(uint32_t) * n));
     VZ_TRY(al((void**)&s->d_fcount, 256));
+    VZ_TRY(al((void**)&s->d_wnodes, sizeof(uint4) * 4 * ni));
+    VZ_TRY(al((void**)&s->d_wkids, sizeof(uint32_t) * 4 * ni));
+    VZ_TRY(al((void**)&s->d_wflag, sizeof(uint32_t) * ni));
+    VZ_TRY(al((void**)&s->d_widx, sizeof(uint32_t) * (ni + blocks_for(ni, kScanTile) + 1)));  // + tile sums
+    VZ_TRY(al((void**)&s->d_wfront[0], sizeof(uint32_t) * ni));
+    VZ_TRY(al((void**)&s->d_wfront[1], sizeof(uint32_t) * ni));
+    VZ_TRY(al((void**)&s->d_wcnt, sizeof(uint32_t) * (kMaxDepth + 8)));
     s->device_bytes = bytes;
     return VZ_SUCCESS;
 }
@@ -636,9 +828,43 @@ void lbvh_free(vzScene s)
     void* ptrs[] = {s->d_sph_in, s->d_mat_in, s->d_keys[0], s->d_keys[1], s->d_vals[0], s->d_vals[1],
                     s->d_hist, s->d_bounds, s->d_sph, s->d_leaf_info, s->d_mat, s->d_nodes,
                     s->d_boxes, s->d_child, s->d_parent_int, s->d_parent_leaf, s->d_visit,
-                    s->d_height, s->d_frontier, s->d_fcount};
+                    s->d_height, s->d_frontier, s->d_fcount, s->d_wnodes, s->d_wkids, s->d_wflag,
+                    s->d_widx, s->d_wfront[0], s->d_wfront[1], s->d_wcnt};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+}
+
+// The 4-wide tree of the current binary tree (after every build and refit).
+// One launch per wide level: the binary depth bounds the wide depth, and
+// levels past the deepest find an empty frontier.
+static vzStatus collapse_wide(vzScene s, hipStream_t st)
+{
+    const uint32_t n = s->n, ni = n - 1;
+    uint32_t* cnt = s->d_wcnt;  // [0, kMaxDepth]: per level; then nw, the wide depth
+    uint32_t* tsum = s->d_widx + ni;
+    const uint32_t ntile = blocks_for(ni, kScanTile);
+    hipLaunchKernelGGL(k_zero, dim3(blocks_for(ni, kBlk)), dim3(kBlk), 0, st, s->d_wflag, ni);
+    hipLaunchKernelGGL(k_wide_init, dim3(1), dim3(kBlk), 0, st, s->d_wflag, s->d_wfront[0], cnt, kMaxDepth + 8);
+    const uint32_t grid = std::min<uint32_t>(1024u, blocks_for(ni, kBlk));
+    for (uint32_t L = 0; L + 1 < s->depth && L < kMaxDepth; L++)
+        hipLaunchKernelGGL(k_wide_level, dim3(grid), dim3(kBlk), 0, st, s->d_child, s->d_boxes,
+                           s->d_wfront[L & 1u], s->d_wfront[(L + 1) & 1u], cnt, L, s->d_wkids, s->d_wflag,
+                           cnt + kMaxDepth + 2);
+    hipLaunchKernelGGL(k_wscan_tiles, dim3(ntile), dim3(kBlk), 0, st, s->d_wflag, ni, tsum);
+    hipLaunchKernelGGL(k_wscan_top, dim3(1), dim3(kBlk), 0, st, tsum, ntile, cnt + kMaxDepth + 1);
+    hipLaunchKernelGGL(k_wscan_apply, dim3(ntile), dim3(kBlk), 0, st, s->d_wflag, ni, tsum, s->d_widx);
+    hipLaunchKernelGGL(k_wide_pack, dim3(blocks_for(ni, kBlk)), dim3(kBlk), 0, st, s->d_sph, s->d_wkids,
+                       s->d_wflag, s->d_widx, s->d_boxes, ni, s->d_wnodes);
+    VZ_HIP_TRY(hipGetLastError());
+    uint32_t h[2] = {0u, 0u};
+    VZ_HIP_TRY(hipMemcpyAsync(h, cnt + kMaxDepth + 1, sizeof(h), hipMemcpyDeviceToHost, st));
+    VZ_HIP_TRY(hipStreamSynchronize(st));
+    s->nw = h[0];
+    s->wdepth = h[1];
+    if (s->wdepth > kMaxWideDepth)
+        return set_error(VZ_ERROR_CAPACITY, "4-wide BVH depth " + std::to_string(s->wdepth) +
+                                                " exceeds the traversal stack (" + std::to_string(kMaxWideDepth) + ")");
+    return VZ_SUCCESS;
 }
 
 static vzStatus refit_core(vzScene s, hipStream_t st)
@@ -662,7 +888,7 @@ static vzStatus refit_core(vzScene s, hipStream_t st)
         return set_error(VZ_ERROR_CAPACITY, "BVH depth " + std::to_string(s->depth) +
                                                 " exceeds the traversal stack (" +
                                                 std::to_string(kMaxDepth) + ")");
-    return VZ_SUCCESS;
+    return collapse_wide(s, st);
 }
 
 vzStatus lbvh_build(vzScene s, hipStream_t st)
@@ -707,7 +933,11 @@ vzStatus lbvh_build(vzScene s, hipStream_t st)
     float ms = 0.0f;
     VZ_HIP_TRY(hipEventElapsedTime(&ms, e0.e, e1.e));
     s->build_ms = ms;
-    if (n < 2) s->depth = 1;
+    if (n < 2) {
+        s->depth = 1;
+        s->nw = 0;
+        s->wdepth = 1;
+    }
     return rs;
 }
 

@@ -1,16 +1,23 @@
-// vz_traverse.h -- closest-hit traversal of the LBVH on gfx950, replacing
+// vz_traverse.h -- closest-hit traversal of the BVH on gfx950, replacing
 // the traversal inside optixTrace (Core/RayTracer.cu:190-202) and the
 // __intersection__hit_sphere program it calls (Core/RayTracer.cu:229-270).
 //
-// Per lane: the ray in registers, a short stack of (node, tnear) in LDS laid
-// out [slot][lane] so a wave's pushes/pops hit distinct banks, and a private
-// overflow array that holds the oldest entries when the LDS ring is full.
-// Nodes are fetched as two 16-byte loads; lanes that visit the same node
-// coalesce into one cache-line access.  Near-first order, leaf children tested
-// as soon as their box is hit, popped entries culled against the current best
-// hit.  Ties resolve to the smaller t, then the smaller primitive id, so the
-// result is the exact lexicographic minimum over all spheres whatever the
-// visit order (the CPU oracle checks this against brute force).
+// The tree walked is the 4-wide one collapsed from the binary LBVH: 64-byte
+// nodes holding four quantised child boxes, so a visit is one node fetch
+// (four 16-byte loads that lanes on the same node share) and four box tests
+// that every lane runs together.  Per lane: the ray in registers, a short
+// stack of (node, entry distance) in LDS laid out [slot][lane] so a wave's
+// pushes and pops hit distinct banks, and a private overflow array that holds
+// the oldest entries when the LDS ring is full.  Per visit: the four slot
+// boxes against [tmin, best]; the leaf slots hit, tested in slot order; then
+// the internal slots hit whose entry distance is still within the best hit,
+// nearest first (lower slot on a tie): the nearest is visited next and the
+// others pushed farthest first.  Popped entries beyond the best hit are
+// dropped.  Ties between hits resolve to the smaller t, then the smaller
+// primitive id, so the result is the exact lexicographic minimum over all
+// spheres whatever the visit order (the CPU oracle checks this against brute
+// force and follows the same visit order, so node-visit and sphere-test
+// counts match it too).
 #pragma once
 #include "vz_device.h"
 #include "vz_internal.h"
@@ -24,26 +31,27 @@ struct TraceResult {
     uint32_t leaf;  // leaf (Morton-order) index, 0xFFFFFFFF on a miss
 };
 
-// Where the 32-byte nodes come from: the scene's array in HBM, or a copy a
-// workgroup staged in LDS (small scenes).  kLeafLoop picks how a visit's two
-// leaf children are tested (see Traversal::leaves), per node source, as
-// measured on the 1080p book scene (LDS) and 1M-sphere box scenes (HBM).
+// Where the 64-byte nodes come from: the scene's array in HBM, or a copy a
+// workgroup staged in LDS (small scenes).
 struct GlobalNodes {
-    static constexpr bool kLeafLoop = false;
     const uint4* p;
-    VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b) const
+    VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b, uint4& c, uint4& d) const
     {
-        a = ldg(p + 2 * i);
-        b = ldg(p + 2 * i + 1);
+        const uint4* q = p + 4 * (size_t)i;
+        a = ldg(q);
+        b = ldg(q + 1);
+        c = ldg(q + 2);
+        d = ldg(q + 3);
     }
 };
 struct LdsNodes {
-    static constexpr bool kLeafLoop = true;
     const uint4* p;  // points into __shared__ memory
-    VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b) const
+    VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b, uint4& c, uint4& d) const
     {
-        a = p[2 * i];
-        b = p[2 * i + 1];
+        a = p[4 * i];
+        b = p[4 * i + 1];
+        c = p[4 * i + 2];
+        d = p[4 * i + 3];
     }
 };
 
@@ -52,24 +60,22 @@ struct LdsNodes {
 // only splits an aggregate into registers when nothing in it is indexed
 // dynamically.
 struct Overflow {
-    uint32_t node[kMaxDepth];
-    float t[kMaxDepth];
+    uint32_t node[kWideOverflow];
+    float t[kWideOverflow];
 };
 
 // Resumable traversal of one ray: init() sets it up, every step() visits
-// one internal node.  A wave can therefore refill a lane with a new ray as
-// soon as its traversal ends instead of waiting for the slowest lane.
+// one wide node.  A wave can therefore refill a lane with a new ray as soon
+// as its traversal ends instead of waiting for the slowest lane.
 // S: LDS slots per lane (power of two); lds_node/lds_t point at this lane's
 // column, consecutive slots `stride` words apart.
 template <int S, bool COUNT>
 struct Traversal {
     v3 o, d;
     float inv[3];
-    uint32_t sel[3];  // per axis: byte selector that puts the near plane first
     float tmin, best;
+    uint32_t negm;  // bit k: the ray runs towards -k, so a slot's near plane on axis k is its hi plane
     uint32_t bleaf, bprim, node, sp, bottom, budget;
-    uint32_t pend, pg;  // the visit waiting for leaves() / decide() (see visit())
-    float ptn0, ptn1;
 
     // Returns true when the ray is already finished (single-sphere scene).
     VZ_DEV bool init(const SceneView& sc, v3 o_, v3 d_, float tmin_, float tmax_, uint32_t& ns)
@@ -81,15 +87,16 @@ struct Traversal {
         bleaf = 0xFFFFFFFFu;
         bprim = 0xFFFFFFFFu;
         const float dd[3] = {d.x, d.y, d.z};
+        negm = 0u;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             float dk = dd[k];
             if (!(fabsf(dk) > 1e-20f)) dk = copysignf(1e-20f, dk);
             inv[k] = 1.0f / dk;
             // lo <= hi and rounding is monotone, so (lo - o) * inv <= (hi - o) * inv
-            // exactly when inv > 0: the near plane of each child is its lo
-            // byte for a positive direction, its hi byte for a negative one
-            sel[k] = inv[k] < 0.0f ? 0x02030001u : 0x03020100u;
+            // exactly when inv > 0: the near plane is lo for a positive
+            // direction, hi for a negative one
+            negm |= (inv[k] < 0.0f ? 1u : 0u) << k;
         }
         if (sc.n == 1u) {
             float t;
@@ -103,11 +110,9 @@ struct Traversal {
         sp = 0;
         bottom = 0;
         node = 0;
-        // a well-formed tree visits each of its n-1 internal nodes at most
-        // once; this and the index guards below only matter for a corrupted
-        // node array
-        budget = sc.n;
-        pend = 0u;
+        // a well-formed tree visits each of its wide nodes at most once; this
+        // and the index guards below only matter for a corrupted node array
+        budget = sc.nw;
         return false;
     }
 
@@ -158,125 +163,11 @@ struct Traversal {
         }
     }
 
-    // A node visit in three parts, so that a wave can run the sphere tests of
-    // many lanes together (Aila & Laine 2009, "while-while"): visit() does the
-    // box tests and records what the node leads to; leaves() tests its leaf
-    // children; decide() picks the next node with the best hit as it stands
-    // after those tests.  Each lane runs the same sequence as step(), so the
-    // result and every count are the same whenever leaves() runs.
-    //
-    // visit(): returns true when the traversal has finished (a guard fired).
-    // Otherwise pend holds the leaf children to test (bits 0-1) and the
-    // internal children to follow (bits 2-3); pg is the node's first child
-    // and ptn the two children's entry distances.
-    template <class NodeSrc>
-    VZ_DEV bool visit(const SceneView& sc, const NodeSrc& nodes, uint32_t& nv)
+    // Pops the nearest stacked node that can still hold a closer hit.
+    // Returns true when none is left (the traversal has finished).
+    VZ_DEV bool pop(uint32_t* lds_node, float* lds_t, uint32_t stride, const Overflow& ov)
     {
-        uint4 A, B;
-        nodes.fetch(node, A, B);
-        if (COUNT) nv++;
-        const float no[3] = {__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z)};
-        const float org[3] = {o.x, o.y, o.z};
-        const uint32_t meta = A.w;
-        const uint32_t qw[3] = {B.x, B.y, B.z};
-        const uint32_t g = B.w;
-        float tn[2];
-        bool hit[2];
-        // per axis, the two children's (lo, hi) bytes reordered to (near, far):
-        // one byte permute replaces a min and a max per child and axis, with
-        // the same result bit for bit
-        uint32_t qs[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) qs[k] = __builtin_amdgcn_perm(qw[k], qw[k], sel[k]);
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            float lo_t = tmin, hi_t = best;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const float s = scale_of(meta, k);
-                const float pn = __builtin_fmaf(s, (float)((qs[k] >> (16 * c)) & 0xFFu), no[k]);
-                const float pf = __builtin_fmaf(s, (float)((qs[k] >> (16 * c + 8)) & 0xFFu), no[k]);
-                lo_t = fmaxf(lo_t, (pn - org[k]) * inv[k]);
-                hi_t = fminf(hi_t, (pf - org[k]) * inv[k]);
-            }
-            tn[c] = lo_t;
-            hit[c] = lo_t <= hi_t * kRobust;
-        }
-        // guards: children must be valid indices, and a well-formed tree
-        // visits each of its n-1 internal nodes at most once.  Folded in as a
-        // select rather than an early return, so that the compiler cannot
-        // move the box arithmetic, and with it the first node load, behind a
-        // wait for the second.
-        const bool bad = g + 1u >= sc.n || budget == 0u;
-        budget--;
-        const uint32_t leaf = (meta >> 24) & 3u;
-        const uint32_t h = (hit[0] ? 1u : 0u) | (hit[1] ? 2u : 0u);
-        pend = bad ? 0u : (h & leaf) | ((h & ~leaf & 3u) << 2);
-        pg = g;
-        ptn0 = tn[0];
-        ptn1 = tn[1];
-        return bad;
-    }
-    VZ_DEV bool has_leaves() const { return (pend & 3u) != 0u; }
-
-    // Sphere tests of the pending visit's leaf children, in child order.  Both
-    // children's spheres come in one load: pg + 1 < n holds after a visit that
-    // did not end the traversal.  LOOP: one copy of the test run once per
-    // pending leaf, so a wave whose lanes hold leaves at different child
-    // positions runs it once, not twice (faster with LDS nodes); otherwise a
-    // copy per child (faster with HBM nodes, whose waits it overlaps).
-    template <bool LOOP>
-    VZ_DEV void leaves(const SceneView& sc, uint32_t& ns)
-    {
-        if ((pend & 3u) == 0u) return;
-        float4 s0, s1;
-        ldg2(sc.spheres + pg, s0, s1);
-        if (LOOP) {
-            uint32_t p = pend & 3u;
-#pragma clang loop unroll(disable)
-            do {
-                const bool first = (p & 1u) != 0u;
-                test_leaf(sc, first ? pg : pg + 1u, first ? s0 : s1, ns);
-                p &= p - 1u;
-            } while (p != 0u);
-        } else {
-            if (pend & 1u) test_leaf(sc, pg, s0, ns);
-            if (pend & 2u) test_leaf(sc, pg + 1u, s1, ns);
-        }
-    }
-
-    // Next node after the pending visit: the near internal child (the far one
-    // pushed), else the only one, else the nearest stacked node that can
-    // still hold a closer hit.  Returns true when the traversal has finished.
-    VZ_DEV bool decide(const SceneView& sc, uint32_t* lds_node, float* lds_t, uint32_t stride, Overflow& ov)
-    {
-        const bool in0 = (pend & 4u) != 0u, in1 = (pend & 8u) != 0u;
-        const uint32_t g = pg;
-        pend = 0u;
-        if (in1 && g + 2u >= sc.n) return true;  // internal child g+1 must be <= n-2
-        if (in0 && in1) {
-            const int nr = (ptn1 < ptn0) ? 1 : 0;
-            const uint32_t far_node = g + (uint32_t)(1 - nr);
-            const float far_t = nr ? ptn0 : ptn1;
-            if (sp - bottom == (uint32_t)S) {  // ring full: oldest entry to the overflow
-                if (bottom >= kMaxDepth) return true;
-                ov.node[bottom] = lds_node[(bottom & (S - 1)) * stride];
-                ov.t[bottom] = lds_t[(bottom & (S - 1)) * stride];
-                bottom++;
-            }
-            lds_node[(sp & (S - 1)) * stride] = far_node;
-            lds_t[(sp & (S - 1)) * stride] = far_t;
-            sp++;
-            node = g + (uint32_t)nr;
-            return false;
-        } else if (in0) {
-            node = g;
-            return false;
-        } else if (in1) {
-            node = g + 1u;
-            return false;
-        }
-        while (sp > 0u) {  // pop, culling entries beyond the best hit
+        while (sp > 0u) {
             sp--;
             uint32_t nd;
             float t;
@@ -296,14 +187,104 @@ struct Traversal {
         return true;
     }
 
-    // One whole node visit; returns true when the traversal has finished.
+    // One wide node visit; returns true when the traversal has finished.
     template <class NodeSrc>
     VZ_DEV bool step(const SceneView& sc, const NodeSrc& nodes, uint32_t* lds_node, float* lds_t,
                      uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns)
     {
-        if (visit(sc, nodes, nv)) return true;
-        leaves<NodeSrc::kLeafLoop>(sc, ns);
-        return decide(sc, lds_node, lds_t, stride, ov);
+        uint4 A, B, C, D;
+        nodes.fetch(node, A, B, C, D);
+        if (COUNT) nv++;
+        const float no[3] = {__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z)};
+        const float org[3] = {o.x, o.y, o.z};
+        const uint32_t meta = A.w;
+        const uint32_t lo_w[3] = {B.x, B.y, B.z}, hi_w[3] = {B.w, C.x, C.y};
+        const uint32_t ch[4] = {C.z, C.w, D.x, D.y};
+        // per axis the four slots' near and far plane bytes: a whole-word
+        // select replaces a min and a max per slot and axis, same result
+        float s[3];
+        uint32_t qn[3], qf[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            s[k] = scale_of(meta, k);
+            const bool ng = ((negm >> k) & 1u) != 0u;
+            qn[k] = ng ? hi_w[k] : lo_w[k];
+            qf[k] = ng ? lo_w[k] : hi_w[k];
+        }
+        float tn[4];
+        uint32_t hitm = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            float lo_t = tmin, hi_t = best;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float pn = __builtin_fmaf(s[k], (float)((qn[k] >> (8 * c)) & 0xFFu), no[k]);
+                const float pf = __builtin_fmaf(s[k], (float)((qf[k] >> (8 * c)) & 0xFFu), no[k]);
+                lo_t = fmaxf(lo_t, (pn - org[k]) * inv[k]);
+                hi_t = fminf(hi_t, (pf - org[k]) * inv[k]);
+            }
+            tn[c] = lo_t;
+            hitm |= (lo_t <= hi_t * kRobust ? 1u : 0u) << c;
+        }
+        hitm &= meta >> 28;
+        const uint32_t leafm = (meta >> 24) & 0xFu;
+        if (budget == 0u) return true;
+        budget--;
+        // leaf slots that were hit, in slot order
+        uint32_t lm = hitm & leafm;
+        while (lm) {
+            const uint32_t c = (uint32_t)__builtin_ctz(lm);
+            const uint32_t leaf = c == 0u ? ch[0] : (c == 1u ? ch[1] : (c == 2u ? ch[2] : ch[3]));
+            if (leaf < sc.n) test_leaf(sc, leaf, ldg(sc.spheres + leaf), ns);
+            lm &= lm - 1u;
+        }
+        // internal slots that were hit and can still hold a closer hit
+        const float lim = best * kRobust;
+        const uint32_t im = hitm & ~leafm;
+        bool cand[4];
+        uint32_t m = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            cand[c] = ((im >> c) & 1u) != 0u && tn[c] <= lim && ch[c] < sc.nw;
+            m += cand[c] ? 1u : 0u;
+        }
+        if (m == 0u) return pop(lds_node, lds_t, stride, ov);
+        // rank of each candidate in (entry distance, slot) order
+        uint32_t rk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = a + 1; b < 4; b++) {
+                const bool both = cand[a] && cand[b];
+                const bool le = tn[a] <= tn[b];
+                rk[b] += (both && le) ? 1u : 0u;
+                rk[a] += (both && !le) ? 1u : 0u;
+            }
+        uint32_t next = 0u;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (cand[c] && rk[c] == 0u) next = ch[c];
+        if (m > 1u) {
+            // room in the LDS ring for m - 1 entries: the oldest go to the overflow
+            while (sp + (m - 1u) - bottom > (uint32_t)S) {
+                if (bottom >= kWideOverflow) return true;
+                ov.node[bottom] = lds_node[(bottom & (S - 1)) * stride];
+                ov.t[bottom] = lds_t[(bottom & (S - 1)) * stride];
+                bottom++;
+            }
+            // the farthest lands deepest: rank r at sp + (m - 1 - r)
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                if (cand[c] && rk[c] != 0u) {
+                    const uint32_t pos = sp + (m - 1u) - rk[c];
+                    lds_node[(pos & (S - 1)) * stride] = ch[c];
+                    lds_t[(pos & (S - 1)) * stride] = tn[c];
+                }
+            }
+            sp += m - 1u;
+        }
+        node = next;
+        return false;
     }
 };
 
