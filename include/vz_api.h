@@ -155,6 +155,18 @@ vzStatus vzAccelDownloadWide(vzScene scene, void* wide_nodes, vzStream stream);
  * smaller primitive id. */
 vzStatus vzTraceClosest(vzScene scene, const vzRay* rays, uint32_t n, vzHit* hits,
                         vzStream stream, vzStats* stats);
+/* As vzTraceClosest, allocating nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under hipGraph
+ * capture, serving loops and harnesses that forbid hipMalloc).  Its
+ * per-workgroup counters live in a fixed 1.5 MiB array in the library's
+ * device memory (one per GPU, held until the process ends), so it traces at
+ * most 16,777,216 rays per call (VZ_ERROR_CAPACITY above that).  Calls that
+ * overlap on one GPU share that array and race: order them on one stream.
+ * With stats == NULL it only enqueues (capturable); with stats it copies the
+ * counters back and synchronises the stream, and leaves the times 0.  The
+ * current device must be the scene's. */
+vzStatus vzTraceClosestNoAlloc(vzScene scene, const vzRay* rays, uint32_t n, vzHit* hits,
+                               vzStream stream, vzStats* stats);
 /* One frame: the OptiX launch of Renderer::Draw (Renderer.h:75) plus the
  * accumulation of __raygen__rg (RayTracer.cu:163-217).  Synchronises the
  * stream when stats != NULL, otherwise returns after enqueueing. */

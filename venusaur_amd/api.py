@@ -115,6 +115,7 @@ def _declare(L):
         "vzAccelDownload": (st, [vp, vp, vp, vp, vp]),
         "vzAccelDownloadWide": (st, [vp, vp, vp]),
         "vzTraceClosest": (st, [vp, vp, u32, vp, vp, P(Stats)]),
+        "vzTraceClosestNoAlloc": (st, [vp, vp, u32, vp, vp, P(Stats)]),
         "vzLaunch": (st, [vp, P(LaunchParams), vp, P(Stats)]),
         "vzBatchLaunch": (st, [vp, P(BatchItem), u32, P(Tuning), vp, P(Stats)]),
         "vzAutotune": (st, [vp, P(BatchItem), u32, vp, P(Tuning)]),
@@ -396,6 +397,13 @@ class Scene:
         st = Stats()
         check(lib().vzTraceClosest(self.h, rays_dev_ptr, n, hits_dev_ptr, stream,
                                    C.byref(st) if stats else None))
+        return st
+
+    def trace_noalloc(self, rays_dev_ptr, n, hits_dev_ptr, stream=None, stats=False):
+        """vzTraceClosestNoAlloc: the same query with the library's fixed scratch."""
+        st = Stats()
+        check(lib().vzTraceClosestNoAlloc(self.h, rays_dev_ptr, n, hits_dev_ptr, stream,
+                                          C.byref(st) if stats else None))
         return st
 
     def launch(self, params, stream=None):

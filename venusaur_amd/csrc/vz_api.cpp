@@ -115,8 +115,11 @@ vzStatus vzCreate(int device, vzContext* out)
     VZ_HIP_TRY(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return set_error(VZ_ERROR_NO_DEVICE, std::string("libvz is built for gfx950, device is ") + prop.gcnArchName);
+    unsigned long long* scratch = nullptr;
+    VZ_TRY(trace_scratch_address(&scratch));  // this device's copy, looked up once with the device current
     vzContext c = new vzContext_st();
     c->device = device;
+    c->trace_scratch = scratch;
     c->num_cus = (uint32_t)prop.multiProcessorCount;
     c->hbm_bytes = (uint64_t)prop.totalGlobalMem;
     // the marketing name can be empty (no entry in the driver's device-id
@@ -324,18 +327,20 @@ vzStatus vzTraceClosest(vzScene scene, const vzRay* rays, uint32_t n, vzHit* hit
 {
     if (!scene || (n && (!rays || !hits))) return set_error(VZ_ERROR_INVALID_VALUE, "NULL argument");
     hipStream_t st = as_stream(stream);
+    // per-workgroup partial counters, then the 4 summed counters
+    const size_t nparts = (size_t)trace_blocks(n) * kTraceParts;
     DevCounters c;
-    VZ_HIP_TRY(hipMalloc(&c.d, 8 * sizeof(unsigned long long)));
-    VZ_HIP_TRY(hipMemsetAsync(c.d, 0, 8 * sizeof(unsigned long long), st));
+    VZ_HIP_TRY(hipMalloc(&c.d, (nparts + 8) * sizeof(unsigned long long)));
     Event e0, e1;
     VZ_EVENT_TRY(e0);
     VZ_EVENT_TRY(e1);
     VZ_HIP_TRY(hipEventRecord(e0.e, st));
-    vzStatus rs = trace_rays(scene->view(), rays, n, hits, st, stats != nullptr, c.d);
+    vzStatus rs = trace_rays(scene->view(), rays, n, hits, st, stats != nullptr, c.d, c.d + nparts);
     unsigned long long h[8] = {};
     if (rs == VZ_SUCCESS && hipEventRecord(e1.e, st) != hipSuccess)
         rs = set_error(VZ_ERROR_HIP, "hipEventRecord failed");
-    if (rs == VZ_SUCCESS && stats && hipMemcpyAsync(h, c.d, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess)
+    if (rs == VZ_SUCCESS && stats &&
+        hipMemcpyAsync(h, c.d + nparts, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st) != hipSuccess)
         rs = set_error(VZ_ERROR_HIP, "hipMemcpyAsync of the trace counters failed");
     // c.d is freed on return: the kernel must be done with it, on every path
     if (hipStreamSynchronize(st) != hipSuccess && rs == VZ_SUCCESS)
@@ -350,6 +355,38 @@ vzStatus vzTraceClosest(vzScene scene, const vzRay* rays, uint32_t n, vzHit* hit
         stats->sphere_tests = h[3];
         stats->render_ms = ms;
         stats->trace_ms = ms;
+        stats->trace_launches = 1;
+    }
+    return VZ_SUCCESS;
+}
+
+vzStatus vzTraceClosestNoAlloc(vzScene scene, const vzRay* rays, uint32_t n, vzHit* hits, vzStream stream,
+                               vzStats* stats)
+{
+    if (!scene || (n && (!rays || !hits))) return set_error(VZ_ERROR_INVALID_VALUE, "NULL argument");
+    if (n > kTraceScratchBlocks * kTraceBlockSize)
+        return set_error(VZ_ERROR_CAPACITY, "vzTraceClosestNoAlloc traces at most " +
+                                                std::to_string(kTraceScratchBlocks * kTraceBlockSize) +
+                                                " rays per call (its fixed scratch); split the call or use "
+                                                "vzTraceClosest");
+    vzContext ctx = scene->ctx;
+    int cur = -1;
+    VZ_HIP_TRY(hipGetDevice(&cur));
+    if (cur != ctx->device)
+        return set_error(VZ_ERROR_INVALID_VALUE, "the current device is not the scene's device");
+    if (!ctx->trace_scratch) return set_error(VZ_ERROR_INTERNAL, "trace scratch address not set");
+    hipStream_t st = as_stream(stream);
+    unsigned long long* parts = ctx->trace_scratch;
+    unsigned long long* counters = parts + (size_t)kTraceScratchBlocks * kTraceParts;
+    VZ_TRY(trace_rays(scene->view(), rays, n, hits, st, stats != nullptr, parts, counters));
+    if (stats) {
+        unsigned long long h[4] = {};
+        VZ_HIP_TRY(hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, st));
+        VZ_HIP_TRY(hipStreamSynchronize(st));
+        std::memset(stats, 0, sizeof(*stats));
+        stats->segments = h[0];
+        stats->node_visits = h[2];
+        stats->sphere_tests = h[3];
         stats->trace_launches = 1;
     }
     return VZ_SUCCESS;

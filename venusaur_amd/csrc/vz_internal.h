@@ -26,6 +26,12 @@ constexpr uint32_t kWideStackS = 8;
 constexpr uint32_t kWideOverflow = 96;
 constexpr uint32_t kMaxWideDepth = (kWideOverflow + kWideStackS) / 3 + 1;
 constexpr uint32_t kMaxRequests = 1024;  // device request-table size
+// vzTraceClosest: workgroup size, counters per workgroup (segments, node
+// visits, sphere tests), and the workgroups the no-allocation entry point's
+// fixed partials array holds (rays per call <= 65536 x 256).
+constexpr uint32_t kTraceBlockSize = 256;
+constexpr int kTraceParts = 3;
+constexpr uint32_t kTraceScratchBlocks = 65536;
 
 // Records a message for vzGetLastError and returns the status.
 vzStatus set_error(vzStatus s, const std::string& msg);
@@ -101,6 +107,7 @@ struct vzContext_st {
     std::mutex mu;
     vz::Engine* engine = nullptr;  // lazily created wavefront engine
     std::atomic<uint32_t> live_scenes{0}, live_servers{0};  // vzDestroy refuses while any remain
+    unsigned long long* trace_scratch = nullptr;  // this device's fixed trace scratch (vzTraceClosestNoAlloc)
 };
 
 struct vzScene_st {
@@ -176,8 +183,16 @@ vzStatus lbvh_check_input(vzScene s, const vzMaterial* mat, hipStream_t st, uint
 void lbvh_free(vzScene s);
 
 // vz_trace.hip
+// Workgroups of a trace of n rays (each writes kTraceParts partial counters).
+uint32_t trace_blocks(uint32_t n);
+// Closest hits of n rays; parts holds trace_blocks(n) * kTraceParts words,
+// counters 4 words (segments, 0, node visits, sphere tests), both device
+// memory written (not accumulated) by the call.
 vzStatus trace_rays(const SceneView& sc, const vzRay* rays, uint32_t n, vzHit* hits,
-                    hipStream_t st, bool count, unsigned long long* d_counters);
+                    hipStream_t st, bool count, unsigned long long* parts, unsigned long long* counters);
+// Device address of the current device's fixed trace scratch:
+// kTraceScratchBlocks * kTraceParts partial words, then 8 counter words.
+vzStatus trace_scratch_address(unsigned long long** out);
 
 // vz_megakernel.hip
 vzStatus megakernel_launch(const SceneView& sc, const vzLaunchParams& p, hipStream_t st,
