@@ -234,7 +234,8 @@ struct Traversal {
         uint32_t lm = hitm & leafm;
         while (lm) {
             const uint32_t c = (uint32_t)__builtin_ctz(lm);
-            const uint32_t leaf = c == 0u ? ch[0] : (c == 1u ? ch[1] : (c == 2u ? ch[2] : ch[3]));
+            const uint32_t lo = (c & 1u) ? ch[1] : ch[0], hi = (c & 1u) ? ch[3] : ch[2];
+            const uint32_t leaf = (c & 2u) ? hi : lo;  // selects, not branches
             if (leaf < sc.n) test_leaf(sc, leaf, ldg(sc.spheres + leaf), ns);
             lm &= lm - 1u;
         }
@@ -249,16 +250,20 @@ struct Traversal {
             m += cand[c] ? 1u : 0u;
         }
         if (m == 0u) return pop(lds_node, lds_t, stride, ov);
-        // rank of each candidate in (entry distance, slot) order
+        // rank of each candidate in (entry distance, slot) order; the other
+        // slots get an infinite key, which only ever raises their own ranks
+        // (a candidate's key is at most best * kRobust, finite)
+        float key[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) key[c] = cand[c] ? tn[c] : __builtin_huge_valf();
         uint32_t rk[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int b = a + 1; b < 4; b++) {
-                const bool both = cand[a] && cand[b];
-                const bool le = tn[a] <= tn[b];
-                rk[b] += (both && le) ? 1u : 0u;
-                rk[a] += (both && !le) ? 1u : 0u;
+                const bool le = key[a] <= key[b];
+                rk[b] += le ? 1u : 0u;
+                rk[a] += le ? 0u : 1u;
             }
         uint32_t next = 0u;
 #pragma unroll
