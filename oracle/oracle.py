@@ -78,6 +78,7 @@ def lib():
             "vzo_bvh_num_wide_nodes": (u32, [vp]),
             "vzo_bvh_wide_nodes": (vp, [vp]),
             "vzo_bvh_wide_depth": (u32, [vp]),
+            "vzo_wide_box_check": (None, [vp, vp, vp, f32, f32, vp, vp, vp]),
             "vzo_trace_binary": (None, [vp, vp, vp, u32, vp, C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]),
             "vzo_bvh_validate": (C.c_int, [vp, vp]),
@@ -313,3 +314,17 @@ def sky(d):
     out = np.zeros(3, np.float32)
     lib().vzo_sky_f32(_p(d), _p(out))
     return out
+
+
+def wide_box_check(node, o, d, tmin, tmax):
+    """Both slot-box tests of one wide node (a WNODE_DT record) for one ray:
+    ((fast mask, exact mask), fast entry distances, exact entry distances).
+    The traversal's test (fast) must accept every slot the exact-form robust
+    test accepts, with an entry distance no larger."""
+    node = np.ascontiguousarray(np.asarray(node, WNODE_DT).reshape(1))
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    masks = np.zeros(2, np.uint32)
+    tf, te = np.zeros(4, np.float32), np.zeros(4, np.float32)
+    lib().vzo_wide_box_check(_p(node), _p(o), _p(d), float(tmin), float(tmax), _p(masks), _p(tf), _p(te))
+    return (int(masks[0]), int(masks[1])), tf, te

@@ -300,6 +300,19 @@ static inline float vzo_scale_of(uint32_t meta, int k)
 #undef COPYSIGN
 #undef ROBUST
 
+/* Both slot-box tests of one wide node for one ray (float): masks[0] and
+ * tn_fast from the traversal's test, masks[1] and tn_exact from the
+ * exact-form robust test it must contain. */
+void vzo_wide_box_check(const vzo_wnode* w, const float o[3], const float d[3], float tmin, float tmax,
+                        uint32_t masks[2], float tn_fast[4], float tn_exact[4])
+{
+    float inv[3];
+    const float org[3] = {o[0], o[1], o[2]};
+    inv_dir_f32(mk_f32(d[0], d[1], d[2]), inv);
+    masks[0] = wide_box_f32(w, org, inv, tmin, tmax * (1.0f + 0x1p-20f), tn_fast);
+    masks[1] = wide_box_exact_f32(w, org, inv, tmin, tmax, tn_exact);
+}
+
 int vzo_hit_sphere_f32(const float o[3], const float d[3], float tmin, float tmax,
                        const vzo_sphere* s, float* t)
 {

@@ -97,6 +97,12 @@ uint32_t vzo_bvh_depth(const vzo_bvh* b);
 uint32_t vzo_bvh_num_wide_nodes(const vzo_bvh* b);
 const vzo_wnode* vzo_bvh_wide_nodes(const vzo_bvh* b);
 uint32_t vzo_bvh_wide_depth(const vzo_bvh* b);
+/* One wide node's slot boxes against a ray (float): masks[0]/tn_fast by the
+ * traversal's one-fma-per-plane test (planes moved out by a rounding bound),
+ * masks[1]/tn_exact by the exact-form robust slab test on the dequantised
+ * boxes.  The first must contain the second. */
+void vzo_wide_box_check(const vzo_wnode* w, const float o[3], const float d[3], float tmin, float tmax,
+                        uint32_t masks[2], float tn_fast[4], float tn_exact[4]);
 /* Returns 0 when every structural invariant holds (binary and wide trees),
  * else a nonzero code. */
 int vzo_bvh_validate(const vzo_bvh* b, const vzo_sphere* s);

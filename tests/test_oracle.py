@@ -455,3 +455,72 @@ def test_box_scene_bvh_equals_brute_force():
     hb, _, _ = b.trace(rays)
     hf = O.trace_brute(s, rays)
     assert hb.tobytes() == hf.tobytes()
+
+
+def _slot_boxes(node):
+    """Dequantised slot boxes of a wide node: (lo[4,3], hi[4,3], valid[4])."""
+    meta = int(node["meta"])
+    lo, hi = np.zeros((4, 3)), np.zeros((4, 3))
+    for k in range(3):
+        s = math.ldexp(1.0, ((meta >> (8 * k)) & 0xFF) - 127)
+        for c in range(4):
+            lo[c, k] = np.float32(s * ((int(node["qlo"][k]) >> (8 * c)) & 0xFF) + float(node["o"][k]))
+            hi[c, k] = np.float32(s * ((int(node["qhi"][k]) >> (8 * c)) & 0xFF) + float(node["o"][k]))
+    return lo, hi, [(meta >> (28 + c)) & 1 for c in range(4)]
+
+
+@pytest.mark.parametrize("offset", [(0.0, 0.0, 0.0), (5000.0, -3000.0, 7000.0)])
+def test_wide_box_test_contains_exact_form(offset):
+    """The traversal's slot test (one fma per plane, planes moved out by a
+    rounding bound) accepts every slot that the exact-form robust slab test
+    on the dequantised box accepts, with an entry distance no larger, and
+    hardly any more.  Rays start inside, outside and exactly on slot planes,
+    run along, near and off the axes, and end (tmax) at or just short of the
+    exact entry distance; one scene sits far from the origin so that the
+    plane offsets are large against the slot extents."""
+    rng = np.random.default_rng(7)
+    s, _ = O.box_scene(4000, 11)
+    for k, off in enumerate(("cx", "cy", "cz")):
+        s[off] = (s[off].astype(np.float64) + offset[k]).astype(np.float32)
+    nodes = O.BVH(s).wide_nodes()
+    exact_total = extra = 0
+    for ni in rng.choice(len(nodes), 250, replace=False):
+        node = nodes[ni]
+        lo, hi, valid = _slot_boxes(node)
+        for _ in range(12):
+            c = int(rng.choice([i for i in range(4) if valid[i]]))
+            ext = np.maximum(hi[c] - lo[c], 1e-3)
+            kind = rng.integers(4)
+            o = lo[c] + rng.uniform(-1.0, 2.0, 3) * ext
+            if kind == 1:  # on a face of the slot box
+                k = rng.integers(3)
+                o[k] = (lo[c] if rng.integers(2) else hi[c])[k]
+            elif kind == 2:  # far away
+                o = (lo[c] + hi[c]) / 2 + rng.normal(size=3) * 200.0 * ext.max()
+            tgt = lo[c] + rng.uniform(0.0, 1.0, 3) * (hi[c] - lo[c])
+            corner = rng.integers(3) == 0
+            if corner:
+                tgt = np.where(rng.integers(2, size=3) == 1, hi[c], lo[c])
+            d = tgt - o
+            mode = rng.integers(4)
+            if mode == 1:
+                d[rng.integers(3)] = 0.0
+            elif mode == 2:
+                d[rng.integers(3)] = rng.choice([1e-9, -1e-9, 1e-25])
+            if not np.any(d):
+                continue
+            o32, d32 = o.astype(np.float32), d.astype(np.float32)
+            for tmin in (0.001, 0.0):
+                (mf, me), tf, te = O.wide_box_check(node, o32, d32, tmin, 1e16)
+                tmaxes = [1e16] + [float(x) for x in te[[j for j in range(4) if (me >> j) & 1]]]
+                for tmax in tmaxes + [float(np.float32(t * (1 - 2.0 ** -22))) for t in tmaxes[1:]]:
+                    (mf, me), tf, te = O.wide_box_check(node, o32, d32, tmin, tmax)
+                    assert mf & me == me, (ni, o32, d32, tmin, tmax, mf, me)
+                    for j in range(4):
+                        if (me >> j) & 1:
+                            assert tf[j] <= te[j], (ni, j, tf[j], te[j])
+                    if tmax == 1e16 and kind != 1 and mode == 0 and not corner:  # tightness, generic rays
+                        exact_total += bin(me).count("1")
+                        extra += bin(mf & ~me).count("1")
+    assert exact_total > 800
+    assert extra <= 0.01 * exact_total, (extra, exact_total)

@@ -174,17 +174,12 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
         // trace stage: advance every busy lane until enough of them finish.
         // (Parking lanes whose node has leaf children, to run the sphere tests
         // of many lanes together, was measured slower at every threshold.)
-        if (refill <= 1u) {  // the whole wave finishes: a plain divergent loop
-            while (active && !tdone) {
-                tdone = step_node(tr, sc, s_bvh, s_node + tid, s_t + tid, ov, nv, ns);
-            }
-        } else {
-            for (;;) {
-                if (active && !tdone) tdone = step_node(tr, sc, s_bvh, s_node + tid, s_t + tid, ov, nv, ns);
-                const uint64_t busy = __ballot(active && !tdone);
-                if (busy == 0ull) break;
-                if (!exhausted && (uint32_t)__popcll(busy) < refill) break;
-            }
+        uint64_t busy = __ballot(active && !tdone);
+        while (busy) {
+            if (active && !tdone) tdone = step_node(tr, sc, s_bvh, s_node + tid, s_t + tid, ov, nv, ns);
+            busy = __ballot(active && !tdone);
+            // refill <= 1: the whole wave finishes before going back for rays
+            if (!exhausted && (uint32_t)__popcll(busy) < refill) break;
         }
         // miss and shade stages for the lanes whose ray finished
         const bool fin = active && tdone;

@@ -834,6 +834,44 @@ void lbvh_free(vzScene s)
         if (p) (void)hipFree(p);
 }
 
+// Structural check of the packed 4-wide tree, so that traversal needs no
+// per-visit guards: every child index in range, every leaf referenced by
+// exactly one slot, every wide node but the root by exactly one slot and the
+// root by none.  Then a walk from the root reaches each node at most once
+// and ends.  Any violation sets *err.
+__global__ void __launch_bounds__(kBlk) k_wide_refs(const uint4* __restrict__ wnodes, const uint32_t* __restrict__ nwp,
+                                                    uint32_t n, uint32_t* __restrict__ wref,
+                                                    uint32_t* __restrict__ lref, uint32_t* __restrict__ err)
+{
+    const uint32_t nw = *nwp;
+    for (uint32_t i = blockIdx.x * kBlk + threadIdx.x; i < nw; i += gridDim.x * kBlk) {
+        const uint4 a = wnodes[4 * (size_t)i], c = wnodes[4 * (size_t)i + 2], d = wnodes[4 * (size_t)i + 3];
+        const uint32_t valid = a.w >> 28, leafm = (a.w >> 24) & 0xFu;
+        const uint32_t ch[4] = {c.z, c.w, d.x, d.y};
+        for (int k = 0; k < 4; k++) {
+            if (!((valid >> k) & 1u)) continue;
+            if ((leafm >> k) & 1u) {
+                if (ch[k] < n) atomicAdd(&lref[ch[k]], 1u);
+                else *err = 1u;
+            } else {
+                if (ch[k] != 0u && ch[k] < nw) atomicAdd(&wref[ch[k]], 1u);
+                else *err = 1u;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_wide_verify(const uint32_t* __restrict__ nwp, uint32_t n,
+                                                      const uint32_t* __restrict__ wref,
+                                                      const uint32_t* __restrict__ lref, uint32_t* __restrict__ err)
+{
+    const uint32_t nw = *nwp;
+    for (uint32_t i = blockIdx.x * kBlk + threadIdx.x; i < n; i += gridDim.x * kBlk) {
+        if (lref[i] != 1u) *err = 1u;
+        if (i < nw && wref[i] != (i == 0u ? 0u : 1u)) *err = 1u;
+    }
+}
+
 // The 4-wide tree of the current binary tree (after every build and refit).
 // One launch per wide level: the binary depth bounds the wide depth, and
 // levels past the deepest find an empty frontier.
@@ -855,12 +893,23 @@ static vzStatus collapse_wide(vzScene s, hipStream_t st)
     hipLaunchKernelGGL(k_wscan_apply, dim3(ntile), dim3(kBlk), 0, st, s->d_wflag, ni, tsum, s->d_widx);
     hipLaunchKernelGGL(k_wide_pack, dim3(blocks_for(ni, kBlk)), dim3(kBlk), 0, st, s->d_sph, s->d_wkids,
                        s->d_wflag, s->d_widx, s->d_boxes, ni, s->d_wnodes);
+    // reference counts in buffers free at this point: the level fronts and
+    // the refit frontier (both rebuilt by every refit)
+    uint32_t* wref = s->d_wfront[0];
+    uint32_t* lref = s->d_frontier;
+    hipLaunchKernelGGL(k_zero, dim3(blocks_for(ni, kBlk)), dim3(kBlk), 0, st, wref, ni);
+    hipLaunchKernelGGL(k_zero, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, lref, n);
+    hipLaunchKernelGGL(k_wide_refs, dim3(grid), dim3(kBlk), 0, st, s->d_wnodes, cnt + kMaxDepth + 1, n, wref, lref,
+                       cnt + kMaxDepth + 3);
+    hipLaunchKernelGGL(k_wide_verify, dim3(grid), dim3(kBlk), 0, st, cnt + kMaxDepth + 1, n, wref, lref,
+                       cnt + kMaxDepth + 3);
     VZ_HIP_TRY(hipGetLastError());
-    uint32_t h[2] = {0u, 0u};
+    uint32_t h[3] = {0u, 0u, 0u};
     VZ_HIP_TRY(hipMemcpyAsync(h, cnt + kMaxDepth + 1, sizeof(h), hipMemcpyDeviceToHost, st));
     VZ_HIP_TRY(hipStreamSynchronize(st));
     s->nw = h[0];
     s->wdepth = h[1];
+    if (h[2] != 0u) return set_error(VZ_ERROR_INTERNAL, "4-wide BVH failed its structural check");
     if (s->wdepth > kMaxWideDepth)
         return set_error(VZ_ERROR_CAPACITY, "4-wide BVH depth " + std::to_string(s->wdepth) +
                                                 " exceeds the traversal stack (" + std::to_string(kMaxWideDepth) + ")");

@@ -31,8 +31,7 @@ struct TraceResult {
     uint32_t leaf;  // leaf (Morton-order) index, 0xFFFFFFFF on a miss
 };
 
-// Where the 64-byte nodes come from: the scene's array in HBM, or a copy a
-// workgroup staged in LDS (small scenes).
+// Where the 64-byte nodes come from: the scene's array in HBM.
 struct GlobalNodes {
     const uint4* p;
     VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b, uint4& c, uint4& d) const
@@ -42,16 +41,6 @@ struct GlobalNodes {
         b = ldg(q + 1);
         c = ldg(q + 2);
         d = ldg(q + 3);
-    }
-};
-struct LdsNodes {
-    const uint4* p;  // points into __shared__ memory
-    VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b, uint4& c, uint4& d) const
-    {
-        a = p[4 * i];
-        b = p[4 * i + 1];
-        c = p[4 * i + 2];
-        d = p[4 * i + 3];
     }
 };
 
@@ -75,7 +64,7 @@ struct Traversal {
     float inv[3];
     float tmin, best;
     uint32_t negm;  // bit k: the ray runs towards -k, so a slot's near plane on axis k is its hi plane
-    uint32_t bleaf, bprim, node, sp, bottom, budget;
+    uint32_t bleaf, bprim, node, sp, bottom;
 
     // Returns true when the ray is already finished (single-sphere scene).
     VZ_DEV bool init(const SceneView& sc, v3 o_, v3 d_, float tmin_, float tmax_, uint32_t& ns)
@@ -110,9 +99,8 @@ struct Traversal {
         sp = 0;
         bottom = 0;
         node = 0;
-        // a well-formed tree visits each of its wide nodes at most once; this
-        // and the index guards below only matter for a corrupted node array
-        budget = sc.nw;
+        // the build checks every child index and that each node has one
+        // parent, so a walk from the root ends: no guards per visit
         return false;
     }
 
@@ -194,49 +182,70 @@ struct Traversal {
     {
         uint4 A, B, C, D;
         nodes.fetch(node, A, B, C, D);
+        return visit(sc, A, B, C, D, lds_node, lds_t, stride, ov, nv, ns);
+    }
+
+    // The visit of the current node, whose 64 bytes are A..D.
+    VZ_DEV bool visit(const SceneView& sc, const uint4& A, const uint4& B, const uint4& C, const uint4& D,
+                      uint32_t* lds_node, float* lds_t, uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns)
+    {
         if (COUNT) nv++;
         const float no[3] = {__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z)};
         const float org[3] = {o.x, o.y, o.z};
         const uint32_t meta = A.w;
         const uint32_t lo_w[3] = {B.x, B.y, B.z}, hi_w[3] = {B.w, C.x, C.y};
         const uint32_t ch[4] = {C.z, C.w, D.x, D.y};
-        // per axis the four slots' near and far plane bytes: a whole-word
-        // select replaces a min and a max per slot and axis, same result
-        float s[3];
+        // Slab distances of the slots' planes s*q + no (q a plane byte) as
+        // one fma per plane, fma(q, s*inv, (no - o)*inv), moved outwards by
+        // e = (255*s + |no| + |no - o|) * |inv| * 2^-19 (near planes) and 2e
+        // (far planes).  The exact-form distance fl(fl(fl(s*q + no) - o) * inv)
+        // differs from the unmoved fma value by less than 10 u (255*s + |no| +
+        // |no - o|) |inv| (u = 2^-24), so every near distance here is <= the
+        // exact form's and every far distance >= the exact form's times
+        // kRobust: the test passes whenever the exact-form robust test does,
+        // which passes whenever a contained sphere's gate does (monotone
+        // rounding), so hits stay independent of the tree.
+        float a[3], bn[3], bf[3];
         uint32_t qn[3], qf[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            s[k] = scale_of(meta, k);
+            const float s = scale_of(meta, k);
+            const float dd = no[k] - org[k];
+            const float b = dd * inv[k];
+            const float w = (__builtin_fmaf(255.0f, s, fabsf(no[k])) + fabsf(dd)) * fabsf(inv[k]);
+            const float e = fmaxf(w * 0x1p-19f, 0x1p-126f);
+            a[k] = s * inv[k];
+            bn[k] = b - e;
+            bf[k] = __builtin_fmaf(2.0f, e, b);
+            // the near plane is the lo plane for a positive direction: a
+            // whole-word select per axis instead of a min and max per slot
             const bool ng = ((negm >> k) & 1u) != 0u;
             qn[k] = ng ? hi_w[k] : lo_w[k];
             qf[k] = ng ? lo_w[k] : hi_w[k];
         }
+        const float lim_t = best * kRobust;
         float tn[4];
         uint32_t hitm = 0;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            float lo_t = tmin, hi_t = best;
+            float lo_t = tmin, hi_t = lim_t;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float pn = __builtin_fmaf(s[k], (float)((qn[k] >> (8 * c)) & 0xFFu), no[k]);
-                const float pf = __builtin_fmaf(s[k], (float)((qf[k] >> (8 * c)) & 0xFFu), no[k]);
-                lo_t = fmaxf(lo_t, (pn - org[k]) * inv[k]);
-                hi_t = fminf(hi_t, (pf - org[k]) * inv[k]);
+                lo_t = fmaxf(lo_t, __builtin_fmaf((float)((qn[k] >> (8 * c)) & 0xFFu), a[k], bn[k]));
+                hi_t = fminf(hi_t, __builtin_fmaf((float)((qf[k] >> (8 * c)) & 0xFFu), a[k], bf[k]));
             }
             tn[c] = lo_t;
-            hitm |= (lo_t <= hi_t * kRobust ? 1u : 0u) << c;
+            hitm |= (lo_t <= hi_t ? 1u : 0u) << c;
         }
         hitm &= meta >> 28;
         const uint32_t leafm = (meta >> 24) & 0xFu;
-        if (budget == 0u) return true;
-        budget--;
         // leaf slots that were hit, in slot order
         uint32_t lm = hitm & leafm;
         while (lm) {
             const uint32_t c = (uint32_t)__builtin_ctz(lm);
             const uint32_t lo = (c & 1u) ? ch[1] : ch[0], hi = (c & 1u) ? ch[3] : ch[2];
             const uint32_t leaf = (c & 2u) ? hi : lo;  // selects, not branches
-            if (leaf < sc.n) test_leaf(sc, leaf, ldg(sc.spheres + leaf), ns);
+            test_leaf(sc, leaf, ldg(sc.spheres + leaf), ns);
             lm &= lm - 1u;
         }
         // internal slots that were hit and can still hold a closer hit
@@ -246,7 +255,7 @@ struct Traversal {
         uint32_t m = 0;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            cand[c] = ((im >> c) & 1u) != 0u && tn[c] <= lim && ch[c] < sc.nw;
+            cand[c] = ((im >> c) & 1u) != 0u && tn[c] <= lim;
             m += cand[c] ? 1u : 0u;
         }
         if (m == 0u) return pop(lds_node, lds_t, stride, ov);
@@ -256,15 +265,13 @@ struct Traversal {
         float key[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) key[c] = cand[c] ? tn[c] : __builtin_huge_valf();
-        uint32_t rk[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int b = a + 1; b < 4; b++) {
-                const bool le = key[a] <= key[b];
-                rk[b] += le ? 1u : 0u;
-                rk[a] += le ? 0u : 1u;
-            }
+        // per pair a < b one comparison: key[a] <= key[b] ranks b after a,
+        // otherwise a after b
+        const uint32_t l01 = key[0] <= key[1] ? 1u : 0u, l02 = key[0] <= key[2] ? 1u : 0u;
+        const uint32_t l03 = key[0] <= key[3] ? 1u : 0u, l12 = key[1] <= key[2] ? 1u : 0u;
+        const uint32_t l13 = key[1] <= key[3] ? 1u : 0u, l23 = key[2] <= key[3] ? 1u : 0u;
+        const uint32_t rk[4] = {3u - (l01 + l02 + l03), 2u + l01 - (l12 + l13), 1u + l02 + l12 - l23,
+                                l03 + l13 + l23};
         uint32_t next = 0u;
 #pragma unroll
         for (int c = 0; c < 4; c++)
