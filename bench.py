@@ -61,8 +61,8 @@ def parse():
     ap.add_argument("--batch-eager", action="store_true",
                     help="batched form without autotuner or hipGraph, each step launch timed (for profiling)")
     ap.add_argument("--batch-tuning", default="",
-                    help="QUEUE,TW,TH,SPLIT for --batch-eager (default: the context's default tuning), e.g. the "
-                         "autotuner's pick from an unprofiled run")
+                    help="QUEUE,TW,TH,SPLIT[,REFILL] for --batch-eager (default: the context's default tuning), "
+                         "e.g. the autotuner's pick from an unprofiled run")
     ap.add_argument("--c2-frames", type=int, default=1,
                     help="timed frames of the 1M-sphere traversal-bound scene (0 = skip)")
     ap.add_argument("--summarize", metavar="KERNEL",
@@ -307,8 +307,10 @@ def run_batched(ctx, n_scenes, rmse_samples=2, eager=False, eager_tuning=""):
     if eager:
         tuning = ctx.default_tuning()
         if eager_tuning:
-            q, tw, th, sp = (int(x) for x in eager_tuning.split(","))
-            tuning.queue_capacity, tuning.tile_w, tuning.tile_h, tuning.slice_split = q, tw, th, sp
+            v = [int(x) for x in eager_tuning.split(",")]
+            tuning.queue_capacity, tuning.tile_w, tuning.tile_h, tuning.slice_split = v[:4]
+            if len(v) > 4:
+                tuning.refill_lanes = v[4]
         for r in reqs:
             r.params.flags |= api.TIME_STAGES
         items = [(scenes[r.scene_key], r.params) for r in reqs]

@@ -194,10 +194,14 @@ typedef struct vzTuning {
  * hipGraphs); the items share the ray queue.  Synchronises the stream. */
 vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n,
                        const vzTuning* tuning, vzStream stream, vzStats* stats);
-/* Renders the batch twice per candidate tuning (queue capacity 4M/8M/12M/24M
- * rays x camera tiles 32x2/8x8, queue split 4), timing the wavefront only, and returns the
- * candidate with the fastest run; caches the choice per context.  The items'
- * accumulators are written (as by vzBatchLaunch). */
+/* Renders the batch twice per candidate tuning, timing the wavefront only,
+ * and returns the candidate with the fastest run.  Stage 1: queue capacity
+ * 4M/8M/12M/24M rays x camera tiles 32x2/16x4/8x8 (default refill, queue
+ * split 4); stage 2, at the best of those: refill_lanes 25/33/49/57, then
+ * slice_split 1/2.  The choice becomes this context's default and is cached
+ * for the process per kind of device (name and CU count): contexts on the
+ * same kind of device that have not tuned themselves start from it.  The
+ * items' accumulators are written (as by vzBatchLaunch). */
 vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStream stream,
                     vzTuning* out);
 vzStatus vzDefaultTuning(vzContext ctx, vzTuning* out);

@@ -219,8 +219,17 @@ def test_autotune_picks_a_candidate_and_keeps_results(ctx, scenes, oracle_result
     items, tgts = _items(scenes)
     t = api.autotune(ctx, items)
     assert t.queue_capacity in (1 << 22, 1 << 23, 3 << 22, 6 << 22)
-    assert (t.tile_w, t.tile_h) in ((32, 2), (8, 8))
-    assert t.slice_split == 4
+    assert (t.tile_w, t.tile_h) in ((32, 2), (16, 4), (8, 8))
+    assert t.refill_lanes in (0, 25, 33, 49, 57)
+    assert t.slice_split in (1, 2, 4)
+    # cached per kind of device: a fresh context on this GPU starts from it
+    other = api.Context(0)
+    try:
+        d = other.default_tuning()
+        assert (d.queue_capacity, d.tile_w, d.tile_h, d.refill_lanes, d.slice_split) == (
+            t.queue_capacity, t.tile_w, t.tile_h, t.refill_lanes, t.slice_split)
+    finally:
+        other.close()
     for tg in tgts:
         tg.accum.zero()
     api.batch_launch(ctx, items, tuning=t)

@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <random>
 #include <string>
 
@@ -53,10 +55,37 @@ vzStatus check_launch_params(const vzLaunchParams* p)
     return VZ_SUCCESS;
 }
 
+// Autotuned configurations per kind of device (name and CU count), for the
+// life of the process: a context that has not tuned itself starts from the
+// choice another context made on the same kind of device.
+static std::mutex g_tune_mu;
+static std::map<std::string, vzTuning> g_tune_cache;
+
+static std::string device_key(vzContext ctx) { return ctx->name + "/" + std::to_string(ctx->num_cus); }
+
+void tuning_cache_store(vzContext ctx, const vzTuning& t)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[device_key(ctx)] = t;
+}
+
+// Adopts the device's cached tuning if this context has none (ctx->mu held).
+static void adopt_cached_tuning(vzContext ctx)
+{
+    if (ctx->tuned) return;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(device_key(ctx));
+    if (it != g_tune_cache.end()) {
+        ctx->tuning = it->second;
+        ctx->tuned = true;
+    }
+}
+
 static vzStatus get_engine(vzContext ctx, Engine** out)
 {
     if (!ctx->engine) {
         EngineConfig cfg;
+        adopt_cached_tuning(ctx);
         if (ctx->tuned) {
             cfg.queue_capacity = ctx->tuning.queue_capacity;
             cfg.tile_w = ctx->tuning.tile_w;
@@ -471,6 +500,7 @@ vzStatus vzLaunch(vzScene scene, const vzLaunchParams* params, vzStream stream, 
 vzStatus vzDefaultTuning(vzContext ctx, vzTuning* out)
 {
     if (!ctx || !out) return set_error(VZ_ERROR_INVALID_VALUE, "NULL argument");
+    adopt_cached_tuning(ctx);
     if (ctx->tuned) {
         *out = ctx->tuning;
         return VZ_SUCCESS;

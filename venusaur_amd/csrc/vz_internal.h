@@ -194,6 +194,9 @@ vzStatus trace_rays(const SceneView& sc, const vzRay* rays, uint32_t n, vzHit* h
 // kTraceScratchBlocks * kTraceParts partial words, then 8 counter words.
 vzStatus trace_scratch_address(unsigned long long** out);
 
+// vz_api.cpp: the process-wide autotune cache, keyed by device name and CU count
+void tuning_cache_store(vzContext ctx, const vzTuning& t);
+
 // vz_megakernel.hip
 vzStatus megakernel_launch(const SceneView& sc, const vzLaunchParams& p, hipStream_t st,
                            unsigned long long* d_counters, bool count);

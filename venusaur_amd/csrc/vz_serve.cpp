@@ -329,11 +329,25 @@ vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStrea
     // favoured small queues that lost on the real batch.
     std::vector<vzBatchItem> probe(items, items + n);
     for (auto& it : probe) it.params.flags &= ~(uint32_t)(VZ_COUNT_VISITS | VZ_TIME_STAGES);
-    const uint32_t caps[] = {1u << 22, 1u << 23, 3u << 22, 6u << 22};
-    const uint32_t tiles[][2] = {{32, 2}, {8, 8}};
     vzTuning best{};
     double best_rate = -1.0;
-    for (uint32_t cap : caps) {
+    auto time = [&](const vzTuning& cand) -> vzStatus {
+        double rate = 0.0;
+        for (int rep = 0; rep < 2; rep++) {
+            vzStats st{};
+            VZ_TRY(vzBatchLaunch(ctx, probe.data(), n, &cand, stream, &st));
+            if (st.render_ms > 0.0) rate = std::max(rate, (double)st.segments / st.render_ms);
+        }
+        if (rate > best_rate) {
+            best_rate = rate;
+            best = cand;
+        }
+        return VZ_SUCCESS;
+    };
+    // stage 1: queue capacity x camera tile at the default refill and split
+    const uint32_t caps[] = {1u << 22, 1u << 23, 3u << 22, 6u << 22};
+    const uint32_t tiles[][2] = {{32, 2}, {16, 4}, {8, 8}};
+    for (uint32_t cap : caps)
         for (auto& t : tiles) {
             vzTuning cand{};
             cand.queue_capacity = cap;
@@ -342,21 +356,27 @@ vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStrea
             cand.iters_per_graph = 16;
             cand.refill_lanes = 0;
             cand.slice_split = kBatchSplit;
-            double rate = 0.0;
-            for (int rep = 0; rep < 2; rep++) {
-                vzStats st{};
-                VZ_TRY(vzBatchLaunch(ctx, probe.data(), n, &cand, stream, &st));
-                if (st.render_ms > 0.0) rate = std::max(rate, (double)st.segments / st.render_ms);
-            }
-            if (rate > best_rate) {
-                best_rate = rate;
-                best = cand;
-            }
+            VZ_TRY(time(cand));
         }
+    // stage 2: at the best queue and tile, the refill threshold, then the split
+    const vzTuning base = best;
+    for (uint32_t refill : {25u, 33u, 49u, 57u}) {
+        vzTuning cand = base;
+        cand.refill_lanes = refill;
+        VZ_TRY(time(cand));
     }
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    ctx->tuning = best;
-    ctx->tuned = true;
+    const vzTuning base2 = best;
+    for (uint32_t split : {1u, 2u}) {
+        vzTuning cand = base2;
+        cand.slice_split = split;
+        VZ_TRY(time(cand));
+    }
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->tuning = best;
+        ctx->tuned = true;
+    }
+    tuning_cache_store(ctx, best);
     *out = best;
     return VZ_SUCCESS;
 }
