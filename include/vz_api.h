@@ -185,9 +185,9 @@ typedef struct vzTuning {
     uint32_t refill_lanes;    /* a wave fetches new rays once fewer than this many of its lanes
                                  still trace: 1 = only when all are done, 65 = after every
                                  node step, 0 = the default for the scene's node placement */
-    uint32_t slice_split;     /* queue slices per resident step workgroup: 1, 2 or 4 (0 = 4).
-                                 Slices of uneven cost (many scenes in one run) balance better
-                                 when split; vzLaunch always uses 1 */
+    uint32_t slice_split;     /* queue slices per resident step workgroup: 1, 2, 4 or 8 (0 = 4 for
+                                 batches, 8 for servers).  Slices of uneven cost (many scenes
+                                 in one run) balance better when split; vzLaunch always uses 1 */
 } vzTuning;
 
 /* Renders every item in one wavefront run (its iterations replayed as
@@ -198,7 +198,7 @@ vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n,
  * and returns the candidate with the fastest run.  Stage 1: queue capacity
  * 4M/8M/12M/24M rays x camera tiles 32x2/16x4/8x8 (default refill, queue
  * split 4); stage 2, at the best of those: refill_lanes 25/33/49/57, then
- * slice_split 1/2.  The choice becomes this context's default and is cached
+ * slice_split 1/2/8.  The choice becomes this context's default and is cached
  * for the process per kind of device (name and CU count): contexts on the
  * same kind of device that have not tuned themselves start from it.  The
  * items' accumulators are written (as by vzBatchLaunch). */
@@ -213,8 +213,8 @@ typedef struct vzServeConfig {
                                     camera seeds).  Scenes are the caller's and already resident;
                                     they are not counted (vzAccelInfo.device_bytes gives theirs) */
     uint32_t max_active;         /* max requests in the device table (<= 1024) */
-    vzTuning tuning;             /* 0 fields take the serving defaults: 12M-ray queue, 8x8 tiles,
-                                    queue split 4 */
+    vzTuning tuning;             /* 0 fields take the serving defaults: 24M-ray queue, 8x8 tiles,
+                                    queue split 8 */
 } vzServeConfig;
 
 typedef struct vzRequestResult {

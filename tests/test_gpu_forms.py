@@ -78,7 +78,7 @@ def test_batched_alone_equals_mix(ctx, scenes, oracle_results):
 
 
 @pytest.mark.parametrize("q,tw,th,split", [(1 << 16, 8, 8, 4), (1 << 16, 8, 8, 1), (1 << 20, 16, 4, 2),
-                                           (1 << 22, 64, 1, 4), (1 << 23, 32, 2, 1)])
+                                           (1 << 22, 64, 1, 4), (1 << 23, 32, 2, 1), (1 << 22, 8, 8, 8)])
 def test_batched_any_tuning_same_result(ctx, scenes, oracle_results, q, tw, th, split):
     say(f"case: batched tuning q={q} tile={tw}x{th} split={split}")
     items, tgts = _items(scenes)
@@ -221,7 +221,7 @@ def test_autotune_picks_a_candidate_and_keeps_results(ctx, scenes, oracle_result
     assert t.queue_capacity in (1 << 22, 1 << 23, 3 << 22, 6 << 22)
     assert (t.tile_w, t.tile_h) in ((32, 2), (16, 4), (8, 8))
     assert t.refill_lanes in (0, 25, 33, 49, 57)
-    assert t.slice_split in (1, 2, 4)
+    assert t.slice_split in (1, 2, 4, 8)
     # cached per kind of device: a fresh context on this GPU starts from it
     other = api.Context(0)
     try:

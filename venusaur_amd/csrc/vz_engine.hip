@@ -322,9 +322,11 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s, uint32_t* total)
     return r;
 }
 
-constexpr uint32_t kSlicesPerThread = kMaxBlocks / 1024;
 
 
+// SPT: queue slices per control thread (NB <= 1024 * SPT); launched with
+// the smallest that covers the run's slices.
+template <uint32_t SPT>
 __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__ req,
                                                      ReqState* __restrict__ st, Counters* cnt,
                                                      Plan* __restrict__ plan,
@@ -336,15 +338,15 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     const uint32_t r = threadIdx.x;
     const uint32_t nslots = cnt->n_slots;
     const uint32_t iter = cnt->iteration;
-    // 1. free room of the slices this thread owns (kSlicesPerThread * r + q)
+    // 1. free room of the slices this thread owns (SPT * r + q)
     // A run may cap the rays in flight below the queue's capacity: every
     // slice is then filled to Rf <= R (slice counts never exceed Rf, since
     // grants stop there and a step only removes rays).
     const uint32_t Rf = (cnt->fill != 0u && cnt->fill < R) ? cnt->fill : R;
-    uint32_t fr[kSlicesPerThread];
+    uint32_t fr[SPT];
     uint32_t my_free = 0;
-    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
-        const uint32_t sb = r * kSlicesPerThread + q;
+    for (uint32_t q = 0; q < SPT; q++) {
+        const uint32_t sb = r * SPT + q;
         fr[q] = (sb < NB && cnt_next[sb] < Rf) ? Rf - cnt_next[sb] : 0u;
         my_free += fr[q];
     }
@@ -445,16 +447,16 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     // below L gets a share of the new rays proportional to its deficit.
     const unsigned long long held = (unsigned long long)NB * Rf - freeq;
     const uint32_t level = (uint32_t)((held + totR + NB - 1u) / NB);
-    uint32_t def[kSlicesPerThread], my_def = 0;
-    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
+    uint32_t def[SPT], my_def = 0;
+    for (uint32_t q = 0; q < SPT; q++) {
         const uint32_t c = Rf - fr[q];
-        def[q] = (r * kSlicesPerThread + q < NB && level > c) ? level - c : 0u;  // <= fr[q]: level <= Rf
+        def[q] = (r * SPT + q < NB && level > c) ? level - c : 0u;  // <= fr[q]: level <= Rf
         my_def += def[q];
     }
     uint32_t tdef;
     (void)block_excl_scan(my_def, s_scan, &tdef);  // tdef >= totR
-    uint32_t g[kSlicesPerThread], my_g = 0, my_short = 0;
-    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
+    uint32_t g[SPT], my_g = 0, my_short = 0;
+    for (uint32_t q = 0; q < SPT; q++) {
         g[q] = tdef ? (uint32_t)((unsigned long long)totR * def[q] / tdef) : 0u;
         my_g += g[q];
         my_short += g[q] < def[q] ? 1u : 0u;
@@ -466,18 +468,18 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     // the first short slices
     const uint32_t left = totR - tg;
     uint32_t k = short_pre;
-    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
+    for (uint32_t q = 0; q < SPT; q++) {
         if (g[q] < def[q]) {
             if (k < left) g[q]++;
             k++;
         }
     }
     uint32_t my_final = 0;
-    for (uint32_t q = 0; q < kSlicesPerThread; q++) my_final += g[q];
+    for (uint32_t q = 0; q < SPT; q++) my_final += g[q];
     uint32_t tfinal;
     uint32_t run = block_excl_scan(my_final, s_scan, &tfinal);
-    for (uint32_t q = 0; q < kSlicesPerThread; q++) {
-        const uint32_t sb = r * kSlicesPerThread + q;
+    for (uint32_t q = 0; q < SPT; q++) {
+        const uint32_t sb = r * SPT + q;
         if (sb < NB) {
             const uint32_t c = Rf - fr[q];
             plan->blk_first[sb] = run;
@@ -654,7 +656,7 @@ __global__ void __launch_bounds__(kBlk) k_wf_finish(const ReqDesc* __restrict__ 
                                                     float4* __restrict__ n_t,
                                                     BlockStats* __restrict__ bstats, uint32_t R, uint32_t n_apply)
 {
-    __shared__ uint32_t s_first[kMaxBlocks];
+    extern __shared__ uint32_t s_first[];  // NB words (dynamic LDS, sized by the launch)
     __shared__ uint32_t s_paths[kRegenBlk / 64];
     if (blockIdx.x < n_apply)
         apply_passes(req, plan, blockIdx.x, n_apply);
@@ -819,10 +821,15 @@ static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool
     }
 #undef VZ_STEP
     if (ev1) (void)hipEventRecord(ev1, st);
-    hipLaunchKernelGGL(k_wf_control, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt, e->d_plan,
-                       cnt_nxt, e->NB, e->R);
-    hipLaunchKernelGGL(k_wf_finish, dim3(e->grid_small + kRegenOver * e->NB_res), dim3(kBlk), 0, st, e->d_req,
-                       e->d_plan, e->NB, e->q_o[n], e->q_d[n], e->q_t[n], e->d_bstats, e->R, e->grid_small);
+    if (e->NB <= 4096u)
+        hipLaunchKernelGGL(k_wf_control<4>, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt, e->d_plan,
+                           cnt_nxt, e->NB, e->R);
+    else
+        hipLaunchKernelGGL(k_wf_control<kMaxBlocks / 1024>, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state,
+                           e->d_cnt, e->d_plan, cnt_nxt, e->NB, e->R);
+    hipLaunchKernelGGL(k_wf_finish, dim3(e->grid_small + kRegenOver * e->NB_res), dim3(kBlk),
+                       e->NB * sizeof(uint32_t), st, e->d_req, e->d_plan, e->NB, e->q_o[n], e->q_d[n], e->q_t[n],
+                       e->d_bstats, e->R, e->grid_small);
 }
 
 static vzStatus ensure_graph(Engine* e, bool count, bool lds, bool single)
@@ -951,7 +958,7 @@ static vzStatus upload_scenes(Engine* e, const std::vector<const vzScene_st*>& s
 // about the same.
 static void set_split(Engine* e, uint32_t f)
 {
-    f = f >= 4u ? 4u : (f >= 2u ? 2u : 1u);
+    f = f >= 8u ? 8u : (f >= 4u ? 4u : (f >= 2u ? 2u : 1u));
     while (f > 1u && (size_t)e->NB_res * f > kMaxBlocks) f >>= 1;
     e->split = f;
     e->NB = e->NB_res * f;

@@ -219,12 +219,14 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     if (s->cfg.max_active == 0 || s->cfg.max_active > kMaxRequests) s->cfg.max_active = kMaxRequests;
     if (s->cfg.hbm_budget_bytes == 0) s->cfg.hbm_budget_bytes = ctx->hbm_bytes / 2;
     EngineConfig ec;
-    // Serving defaults: a 12M-ray queue, 8x8 camera tiles and the queue split
-    // 4 ways per resident step workgroup.  On the 120-request stream at 3x
-    // the plain capacity: 12M served 3810 Mray-segments/s (p50 0.28 s, p99
-    // 3.95 s) against 3631 (0.31 s, 3.83 s) for 8M and 3798 (0.29 s, 4.15 s)
-    // for 24M; 32x2 tiles at 8M served 3595.
-    ec.queue_capacity = 3u << 22;
+    // Serving defaults: a 24M-ray queue, 8x8 camera tiles and the queue split
+    // 8 ways per resident step workgroup (kServeSplit).  On the 120-request
+    // stream at 3x the plain capacity, 12M had served 3810 Mray-segments/s
+    // against 3631 for 8M and 3798 for 24M (32x2 tiles at 8M: 3595); on the
+    // 2000-request stream, whose 512x512 2048-spp requests hold most of the
+    // work, the larger queue wins (24M at split 8: 5037 against 4699 for 12M
+    // at split 4).
+    ec.queue_capacity = 6u << 22;
     ec.tile_w = 8;
     ec.tile_h = 8;
     if (s->cfg.tuning.queue_capacity) ec.queue_capacity = s->cfg.tuning.queue_capacity;
@@ -235,7 +237,7 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     vzStatus rs = engine_create(ctx, ec, &s->engine);
     if (rs == VZ_SUCCESS && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         rs = set_error(VZ_ERROR_HIP, "hipStreamCreateWithFlags failed");
-    const uint32_t split = s->cfg.tuning.slice_split ? s->cfg.tuning.slice_split : kBatchSplit;
+    const uint32_t split = s->cfg.tuning.slice_split ? s->cfg.tuning.slice_split : kServeSplit;
     if (rs == VZ_SUCCESS) rs = engine_serve_init(s->engine, s->stream, split);
     if (rs != VZ_SUCCESS) {
         if (s->engine) engine_destroy(s->engine);
@@ -366,7 +368,7 @@ vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStrea
         VZ_TRY(time(cand));
     }
     const vzTuning base2 = best;
-    for (uint32_t split : {1u, 2u}) {
+    for (uint32_t split : {1u, 2u, 8u}) {
         vzTuning cand = base2;
         cand.slice_split = split;
         VZ_TRY(time(cand));
