@@ -234,6 +234,7 @@ def _sq(key):
 
 
 NODE_BYTES = 64  # one 4-wide node: four quantised child boxes and the child words
+LDS_WIDE_NODES = 256  # a scene rendered alone with at most this many 4-wide nodes is staged in LDS
 
 
 def survey_bytes(st, node_bytes=NODE_BYTES):
@@ -574,9 +575,10 @@ def main():
 
     # roofline of the step kernel (trace + miss/shade stages, the dominant
     # kernel), from the algorithmic bytes of the counting launch; a scene of
-    # at most 513 spheres rendered alone walks its binary tree (32-byte nodes)
+    # at most LDS_WIDE_NODES 4-wide nodes rendered alone is staged in LDS
     seg_c, nv_c, ns_c = cst.segments, cst.node_visits, cst.sphere_tests
-    nb = 32 if len(spheres) - 1 <= 512 else NODE_BYTES
+    lds_staged = scene.info().num_wide_nodes <= LDS_WIDE_NODES
+    nb = NODE_BYTES
     bytes_step = step_bytes(cst, nb)
     trace_ms = sum(s.trace_ms for s in stats)
     launches = sum(s.trace_launches for s in stats)
@@ -590,20 +592,18 @@ def main():
         # workload: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy
         # kernel of known size (profiles/k_wf_step_book_1080p_256spp.md)
         meas = _traffic("step_kernel_traffic.json", avg_ms) if full else {"traffic": None, "measured_hbm_gbs": None}
-        # the book scene's nodes are staged in LDS, so its node reads (32 B
+        # the book scene's nodes are staged in LDS, so its node reads (64 B
         # per visit) never reach HBM: the HBM-side algorithmic bytes leave
         # them out
-        # a scene of at most 513 spheres rendered alone walks its binary tree
-        # (32-byte nodes) staged in LDS, so those node reads never reach HBM
-        lds_nodes = cst.node_visits if len(spheres) - 1 <= 512 else 0
-        hbm_alg = (bytes_step - 32 * lds_nodes) * a.steps / launches
+        lds_nodes = cst.node_visits if lds_staged else 0
+        hbm_alg = (bytes_step - NODE_BYTES * lds_nodes) * a.steps / launches
         svy = survey_bytes(cst, nb) * a.steps / launches
         hbm = {"peak_gbs": HBM_PEAK_GBS,
                "step_bytes_per_launch": round(per_launch_bytes), "step_bytes_gbs": round(achieved, 1),
                "step_bytes_frac": round(achieved / HBM_PEAK_GBS, 4),
                "survey_formula_bytes_per_segment": round(survey_bytes(cst, nb) / seg_c, 1),
                "survey_formula_frac": round(svy / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-               "node_bytes_from_lds_per_launch": round(32 * lds_nodes * a.steps / launches),
+               "node_bytes_from_lds_per_launch": round(NODE_BYTES * lds_nodes * a.steps / launches),
                "hbm_algorithmic_bytes_per_launch": round(hbm_alg),
                "hbm_algorithmic_frac": round(hbm_alg / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         if meas.get("measured_hbm_gbs"):

@@ -11,7 +11,6 @@
 
 #include "vz_engine.h"
 #include "vz_traverse.h"
-#include "vz_traverse_bin.h"
 
 namespace vz {
 namespace {
@@ -19,7 +18,7 @@ namespace {
 constexpr int kStepBlk = 512;   // step-kernel workgroup (8 waves)
 constexpr int kStackS = kWideStackS;  // LDS stack slots per lane
 constexpr int kTbl = 128;       // LDS (request, pass) tally table entries
-constexpr uint32_t kLdsNodes = 512;  // a scene of at most this many binary nodes is staged in LDS (16 KB)
+constexpr uint32_t kLdsNodes = 256;  // a scene of at most this many 4-wide nodes is staged in LDS (16 KB)
 // A wave goes back for new rays once fewer than this many of its lanes still
 // trace (1 = only when all are done, 65 = after every node step).  Defaults per
 // node source, chosen by sweeping the 1080p book scene (LDS nodes) and
@@ -71,20 +70,13 @@ __device__ __forceinline__ void terminate(const ReqDesc& D, uint32_t slot, uint3
     pb[2u * D.npix + pixel] = L.z;
 }
 
-// One node visit of either traversal: the binary tree staged in LDS, or the
-// 4-wide tree from memory.
-template <int S, bool COUNT>
-__device__ __forceinline__ bool step_node(BinTraversal<S, COUNT>& tr, const SceneView& sc, const uint4* s_bvh,
-                                          uint32_t* lds_node, float* lds_t, BinOverflow& ov, uint32_t& nv,
-                                          uint32_t& ns)
-{
-    return tr.step(sc, LdsBinNodes{s_bvh}, lds_node, lds_t, kStepBlk, ov, nv, ns);
-}
-template <int S, bool COUNT>
-__device__ __forceinline__ bool step_node(Traversal<S, COUNT>& tr, const SceneView& sc, const uint4*,
+// One node visit: the 4-wide tree staged in LDS (LDSN) or read from memory.
+template <bool LDSN, int S, bool COUNT>
+__device__ __forceinline__ bool step_node(Traversal<S, COUNT>& tr, const SceneView& sc, const uint4* s_bvh,
                                           uint32_t* lds_node, float* lds_t, Overflow& ov, uint32_t& nv,
                                           uint32_t& ns)
 {
+    if (LDSN) return tr.step(sc, LdsNodes{s_bvh}, lds_node, lds_t, kStepBlk, ov, nv, ns);
     return tr.step(sc, GlobalNodes{sc.nodes}, lds_node, lds_t, kStepBlk, ov, nv, ns);
 }
 
@@ -114,11 +106,11 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
     __shared__ unsigned long long t_seg[kTbl];
     __shared__ uint32_t s_head, s_out;
     __shared__ unsigned long long s_red[4][kStepBlk / 64];
-    __shared__ uint4 s_bvh[LDSN ? 2 * kLdsNodes : 1];
+    __shared__ uint4 s_bvh[LDSN ? 4 * kLdsNodes : 1];
     const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
     if (LDSN) {  // one small scene: stage its node array in LDS
-        const uint32_t nn = 2u * (scenes[0].n - 1u);
-        const uint4* src = scenes[0].bnodes;
+        const uint32_t nn = 4u * scenes[0].nw;
+        const uint4* src = scenes[0].nodes;
         for (uint32_t i = tid; i < nn; i += kStepBlk) s_bvh[i] = src[i];
     }
     for (uint32_t i = tid; i < (uint32_t)kTbl; i += kStepBlk) {
@@ -137,10 +129,8 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
     uint32_t nv = 0, ns = 0, nh = 0, seg = 0;
     SceneView sc;
     if (SINGLE) sc = scenes[0];
-    // a small scene rendered alone walks its binary tree from LDS; every
-    // other launch walks the 4-wide tree from memory
-    typename std::conditional<LDSN, BinTraversal<kStackS, COUNT>, Traversal<kStackS, COUNT>>::type tr;
-    typename std::conditional<LDSN, BinOverflow, Overflow>::type ov;
+    Traversal<kStackS, COUNT> tr;
+    Overflow ov;
     bool active = false, tdone = false;
     size_t qi = 0;
     uint32_t meta = 0;
@@ -176,7 +166,7 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
         // of many lanes together, was measured slower at every threshold.)
         uint64_t busy = __ballot(active && !tdone);
         while (busy) {
-            if (active && !tdone) tdone = step_node(tr, sc, s_bvh, s_node + tid, s_t + tid, ov, nv, ns);
+            if (active && !tdone) tdone = step_node<LDSN>(tr, sc, s_bvh, s_node + tid, s_t + tid, ov, nv, ns);
             busy = __ballot(active && !tdone);
             // refill <= 1: the whole wave finishes before going back for rays
             if (!exhausted && (uint32_t)__popcll(busy) < refill) break;
@@ -1005,7 +995,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
     const bool use_graph = opt.use_graph && !opt.time_trace;
     // a single small scene is staged in LDS by every step workgroup
     const bool single = scenes.size() == 1;
-    const bool lds = single && scenes[0]->n >= 2 && scenes[0]->n - 1 <= kLdsNodes;
+    const bool lds = single && scenes[0]->n >= 2 && scenes[0]->nw <= kLdsNodes;
     if (use_graph) VZ_TRY(ensure_graph(e, opt.count_visits, lds, single));
     const uint32_t B = e->cfg.iters_per_graph;
     Event t0, t1, blk0(hipEventDisableTiming), blk1(hipEventDisableTiming);

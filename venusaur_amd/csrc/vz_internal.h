@@ -57,7 +57,6 @@ vzStatus check_spheres(const vzSphere* s, uint32_t n);
 
 struct SceneView {           // what kernels need to traverse one scene
     const uint4* nodes;      // 4-wide nodes, 4 uint4 (64 B) each, root = 0
-    const uint4* bnodes;     // binary nodes, 2 uint4 (32 B) each, root = 0 (staged in LDS for small scenes)
     const float4* spheres;   // leaf order {c, r}
     const uint32_t* leaf_info;  // leaf -> prim id | material type << 30
     const float4* mats;      // leaf order {albedo rgb, fuzz | ior}
@@ -160,7 +159,6 @@ struct vzScene_st {
     {
         vz::SceneView v;
         v.nodes = d_wnodes;
-        v.bnodes = d_nodes;
         v.spheres = d_sph;
         v.leaf_info = d_leaf_info;
         v.mats = d_mat;

@@ -31,7 +31,18 @@ struct TraceResult {
     uint32_t leaf;  // leaf (Morton-order) index, 0xFFFFFFFF on a miss
 };
 
-// Where the 64-byte nodes come from: the scene's array in HBM.
+// Where the 64-byte nodes come from: the scene's array in HBM, or a copy a
+// workgroup staged in LDS (a small scene rendered alone).
+struct LdsNodes {
+    const uint4* p;  // points into __shared__ memory
+    VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b, uint4& c, uint4& d) const
+    {
+        a = p[4 * i];
+        b = p[4 * i + 1];
+        c = p[4 * i + 2];
+        d = p[4 * i + 3];
+    }
+};
 struct GlobalNodes {
     const uint4* p;
     VZ_DEV void fetch(uint32_t i, uint4& a, uint4& b, uint4& c, uint4& d) const
