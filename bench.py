@@ -598,11 +598,13 @@ def main():
         lds_nodes = cst.node_visits if lds_staged else 0
         hbm_alg = (bytes_step - NODE_BYTES * lds_nodes) * a.steps / launches
         svy = survey_bytes(cst, nb) * a.steps / launches
-        hbm = {"peak_gbs": HBM_PEAK_GBS,
+        # with the nodes in LDS the algorithmic bytes are not HBM bytes: no
+        # fraction of the HBM peak is given for them
+        hbm = {"peak_gbs": HBM_PEAK_GBS, "node_source": "lds" if lds_staged else "hbm",
                "step_bytes_per_launch": round(per_launch_bytes), "step_bytes_gbs": round(achieved, 1),
-               "step_bytes_frac": round(achieved / HBM_PEAK_GBS, 4),
+               "step_bytes_frac": None if lds_staged else round(achieved / HBM_PEAK_GBS, 4),
                "survey_formula_bytes_per_segment": round(survey_bytes(cst, nb) / seg_c, 1),
-               "survey_formula_frac": round(svy / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               "survey_formula_frac": None if lds_staged else round(svy / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                "node_bytes_from_lds_per_launch": round(NODE_BYTES * lds_nodes * a.steps / launches),
                "hbm_algorithmic_bytes_per_launch": round(hbm_alg),
                "hbm_algorithmic_frac": round(hbm_alg / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
