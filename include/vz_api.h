@@ -101,7 +101,15 @@ typedef struct vzAccelInfo {
     double build_ms;          /* last build/refit time (HIP events) */
     uint32_t num_wide_nodes;  /* nodes of the 4-wide tree traversal uses (64 bytes each) */
     uint32_t wide_depth;      /* its levels from the root to the deepest leaf */
+    uint32_t builder;         /* VZ_BUILD_LBVH, or VZ_BUILD_SAH for 2 to 1024 spheres */
 } vzAccelInfo;
+
+/* Which build made the binary tree.  Scenes of 2 to 1024 spheres get a
+ * binned-SAH tree built top down by one workgroup (leaves in split order,
+ * the morton array then holds each leaf's key unsorted); larger ones an
+ * LBVH (Morton-sorted leaves, Karras linking).  Both use one node layout and
+ * give the same closest hits. */
+enum { VZ_BUILD_LBVH = 0, VZ_BUILD_SAH = 1 };
 
 /* ---- context -------------------------------------------------------- */
 vzStatus vzCreate(int device, vzContext* out);
@@ -114,7 +122,7 @@ vzStatus vzDeviceInfo(vzContext ctx, uint32_t* num_cus, uint64_t* hbm_bytes, cha
 
 /* ---- scene and acceleration structure ------------------------------ */
 /* Copies n spheres/materials from host memory, uploads them and builds the
- * LBVH on the GPU (replaces Renderer.h:160-255 and 452-520). */
+ * BVH on the GPU (replaces Renderer.h:160-255 and 452-520). */
 vzStatus vzSceneCreate(vzContext ctx, const vzSphere* spheres, const vzMaterial* materials,
                        uint32_t n, vzStream stream, vzScene* out);
 /* As vzSceneCreate, with spheres and materials already in device memory
@@ -125,7 +133,7 @@ vzStatus vzSceneCreateDevice(vzContext ctx, const vzSphere* spheres, const vzMat
 /* Fails with VZ_ERROR_INVALID_VALUE while requests that name the scene are
  * queued or running on a server (vzServerSubmit); the scene is unchanged. */
 vzStatus vzSceneDestroy(vzScene scene);
-/* Rebuild the LBVH from the current spheres. */
+/* Rebuild the BVH from the current spheres. */
 vzStatus vzAccelBuild(vzScene scene, vzStream stream);
 /* Replace sphere geometry (same count, same materials) and refit the
  * existing topology: boxes and node quantisation are recomputed bottom-up. */

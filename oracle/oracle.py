@@ -22,6 +22,8 @@ NODE_DT = np.dtype([("o", "<f4", 3), ("meta", "<u4"), ("q", "<u4", 3), ("child",
 WNODE_DT = np.dtype([("o", "<f4", 3), ("meta", "<u4"), ("qlo", "<u4", 3), ("qhi", "<u4", 3),
                      ("child", "<u4", 4), ("pad", "<u4", 2)])
 MISS = 0xFFFFFFFF
+BUILD_LBVH, BUILD_SAH = 0, 1
+SAH_MAX_PRIMS = 1024
 
 
 class Camera(C.Structure):
@@ -78,6 +80,7 @@ def lib():
             "vzo_bvh_num_wide_nodes": (u32, [vp]),
             "vzo_bvh_wide_nodes": (vp, [vp]),
             "vzo_bvh_wide_depth": (u32, [vp]),
+            "vzo_bvh_builder": (u32, [vp]),
             "vzo_wide_box_check": (None, [vp, vp, vp, f32, f32, vp, vp, vp]),
             "vzo_trace_binary": (None, [vp, vp, vp, u32, vp, C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]),
@@ -204,6 +207,10 @@ class BVH:
 
     def wide_depth(self):
         return lib().vzo_bvh_wide_depth(self.h)
+
+    def builder(self):
+        """BUILD_LBVH, or BUILD_SAH for 2 to 1024 spheres."""
+        return lib().vzo_bvh_builder(self.h)
 
     def validate(self):
         return lib().vzo_bvh_validate(self.h, _p(self.spheres))

@@ -29,6 +29,7 @@ struct vzo_bvh {
     uint32_t nw;          /* wide nodes */
     vzo_wnode* wnodes;
     uint32_t wdepth;      /* wide levels from the root to the deepest leaf */
+    uint32_t builder;     /* VZO_BUILD_LBVH or VZO_BUILD_SAH */
 };
 
 /* ---------------------------------------------------------------- RNG --- */
@@ -650,6 +651,141 @@ static void collapse_wide(vzo_bvh* b, const vzo_sphere* s, const uint32_t* gam, 
     free(queue);
 }
 
+/* ---- binned-SAH build of a small scene (n <= VZO_SAH_MAX_PRIMS) -------------
+ * Top down over ranges of a leaf permutation, root range [0, n-1].  A range
+ * [a, b] at depth d < VZO_SAH_DEPTH_CAP is split by the binned surface-area
+ * heuristic: per axis k whose centroid extent ext = hi - lo over the range is
+ * positive, each sphere falls in bin min(NB - 1, (int)(((c_k - lo) / ext) *
+ * NB)) (NB = VZO_SAH_BINS); with L and R the unions of the leaf boxes of bins
+ * <= j and > j and nL, nR their sphere counts, the split after bin j costs
+ * area(L) * nL + area(R) * nR (area = dx*dy + dy*dz + dz*dx), over splits with
+ * both sides non-empty.  The cheapest wins, the lower axis and then the lower
+ * bin on a tie; the range is partitioned stably (each side keeps its order).
+ * With no axis of positive extent, or at depth >= VZO_SAH_DEPTH_CAP, the
+ * range splits after its first floor(m/2) spheres as they stand.  The depth
+ * cap bounds the tree: at most VZO_SAH_DEPTH_CAP + 10 internal levels for
+ * 1024 spheres, so at most 35 wide levels, what the GPU traversal's stack
+ * holds.
+ * Nodes use the Karras layout, so every later stage (boxes, quantisation,
+ * refit, collapse) is shared with the LBVH: the split after leaf position g
+ * makes the children g (covering [a, g]) and g + 1 (covering [g + 1, b]),
+ * each a leaf when its range has one sphere, and the root is node 0. */
+#define VZO_SAH_MAX_PRIMS 1024u
+#define VZO_SAH_BINS 32
+#define VZO_SAH_DEPTH_CAP 24u
+
+typedef struct {
+    const vzo_sphere* s;
+    uint32_t* perm;  /* leaf position -> primitive */
+    uint32_t* tmp;
+    uint32_t *gam, *flg, *parent_int, *parent_leaf;
+} sah_ctx;
+
+static inline float sah_cen(const vzo_sphere* s, int k) { return k == 0 ? s->cx : (k == 1 ? s->cy : s->cz); }
+
+static inline int sah_bin(float c, float lo, float ext)
+{
+    int j = (int)(((c - lo) / ext) * (float)VZO_SAH_BINS);
+    return j > VZO_SAH_BINS - 1 ? VZO_SAH_BINS - 1 : j;
+}
+
+static void sah_grow(float B[6], const float b[6])
+{
+    for (int k = 0; k < 3; k++) {
+        B[k] = fminf(B[k], b[k]);
+        B[3 + k] = fmaxf(B[3 + k], b[3 + k]);
+    }
+}
+
+static void sah_range(sah_ctx* c, uint32_t a, uint32_t b, uint32_t self, uint32_t depth)
+{
+    const uint32_t m = b - a + 1;
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (uint32_t i = a; i <= b; i++)
+        for (int k = 0; k < 3; k++) {
+            const float v = sah_cen(&c->s[c->perm[i]], k);
+            lo[k] = fminf(lo[k], v);
+            hi[k] = fmaxf(hi[k], v);
+        }
+    int bk = -1, bj = -1;
+    float bcost = 0.0f;
+    if (depth < VZO_SAH_DEPTH_CAP) {
+        for (int k = 0; k < 3; k++) {
+            const float ext = hi[k] - lo[k];
+            if (!(ext > 0.0f)) continue;
+            float box[VZO_SAH_BINS][6];
+            uint32_t cnt[VZO_SAH_BINS];
+            for (int j = 0; j < VZO_SAH_BINS; j++) {
+                for (int q = 0; q < 3; q++) {
+                    box[j][q] = 3.4e38f;
+                    box[j][3 + q] = -3.4e38f;
+                }
+                cnt[j] = 0;
+            }
+            for (uint32_t i = a; i <= b; i++) {
+                const vzo_sphere* sp = &c->s[c->perm[i]];
+                const int j = sah_bin(sah_cen(sp, k), lo[k], ext);
+                float lb[6];
+                leaf_box(sp, lb);
+                sah_grow(box[j], lb);
+                cnt[j]++;
+            }
+            float ra[VZO_SAH_BINS], R[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
+            uint32_t rn[VZO_SAH_BINS], nr = 0;
+            for (int j = VZO_SAH_BINS - 1; j >= 1; j--) {
+                sah_grow(R, box[j]);
+                nr += cnt[j];
+                ra[j] = box_area(R);
+                rn[j] = nr;
+            }
+            float L[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
+            uint32_t nl = 0;
+            for (int j = 0; j < VZO_SAH_BINS - 1; j++) {
+                sah_grow(L, box[j]);
+                nl += cnt[j];
+                if (nl == 0 || rn[j + 1] == 0) continue;
+                const float cost = box_area(L) * (float)nl + ra[j + 1] * (float)rn[j + 1];
+                if (bk < 0 || cost < bcost) {
+                    bk = k;
+                    bj = j;
+                    bcost = cost;
+                }
+            }
+        }
+    }
+    uint32_t g;
+    if (bk < 0) {
+        g = a + m / 2 - 1;
+    } else {
+        const float ext = hi[bk] - lo[bk];
+        uint32_t nl = 0, w = a;
+        for (uint32_t i = a; i <= b; i++)
+            if (sah_bin(sah_cen(&c->s[c->perm[i]], bk), lo[bk], ext) <= bj) c->tmp[w++] = c->perm[i];
+        nl = w - a;
+        for (uint32_t i = a; i <= b; i++)
+            if (sah_bin(sah_cen(&c->s[c->perm[i]], bk), lo[bk], ext) > bj) c->tmp[w++] = c->perm[i];
+        memcpy(&c->perm[a], &c->tmp[a], sizeof(uint32_t) * m);
+        g = a + nl - 1;
+    }
+    uint32_t f = 0;
+    if (g == a) {
+        f |= 1u;
+        c->parent_leaf[g] = self;
+    } else {
+        c->parent_int[g] = self;
+    }
+    if (g + 1 == b) {
+        f |= 2u;
+        c->parent_leaf[g + 1] = self;
+    } else {
+        c->parent_int[g + 1] = self;
+    }
+    c->gam[self] = g;
+    c->flg[self] = f;
+    if (g > a) sah_range(c, a, g, g, depth + 1);
+    if (g + 1 < b) sah_range(c, g + 1, b, g + 1, depth + 1);
+}
+
 vzo_bvh* vzo_bvh_build(const vzo_sphere* s, uint32_t n)
 {
     if (n == 0) return NULL;
@@ -679,11 +815,11 @@ vzo_bvh* vzo_bvh_build(const vzo_sphere* s, uint32_t n)
                                             : morton_of(&s[i], loB, scB);
         b->leaf_prim[i] = i;
     }
-    radix_sort(b->morton, b->leaf_prim, n);
     b->depth = 1;
+    b->builder = (n >= 2 && n <= VZO_SAH_MAX_PRIMS) ? VZO_BUILD_SAH : VZO_BUILD_LBVH;
+    if (b->builder == VZO_BUILD_LBVH) radix_sort(b->morton, b->leaf_prim, n);
     if (n == 1) return b;
 
-    /* Karras 2012, algorithm 4 */
     uint32_t* parent_int = (uint32_t*)malloc(sizeof(uint32_t) * ni);
     uint32_t* parent_leaf = (uint32_t*)malloc(sizeof(uint32_t) * n);
     uint32_t* gam = (uint32_t*)malloc(sizeof(uint32_t) * ni);
@@ -691,6 +827,17 @@ vzo_bvh* vzo_bvh_build(const vzo_sphere* s, uint32_t n)
     const uint32_t* K = b->morton;
     int nn = (int)n;
     parent_int[0] = 0xFFFFFFFFu;
+    if (b->builder == VZO_BUILD_SAH) {
+        /* leaves in split order; morton[i] keeps the key of leaf i's sphere */
+        uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * n);
+        sah_ctx c = {s, b->leaf_prim, tmp, gam, flg, parent_int, parent_leaf};
+        sah_range(&c, 0, n - 1, 0, 0);
+        for (uint32_t i = 0; i < n; i++) tmp[i] = b->morton[b->leaf_prim[i]];
+        memcpy(b->morton, tmp, sizeof(uint32_t) * n);
+        free(tmp);
+        nn = 0;  /* no Karras linking */
+    }
+    /* Karras 2012, algorithm 4 */
     for (int i = 0; i < nn - 1; i++) {
         int d = (kdelta(K, nn, i, i + 1) - kdelta(K, nn, i, i - 1)) > 0 ? 1 : -1;
         int dmin = kdelta(K, nn, i, i - d);
@@ -803,6 +950,7 @@ uint32_t vzo_bvh_depth(const vzo_bvh* b) { return b->depth; }
 uint32_t vzo_bvh_num_wide_nodes(const vzo_bvh* b) { return b->nw; }
 const vzo_wnode* vzo_bvh_wide_nodes(const vzo_bvh* b) { return b->wnodes; }
 uint32_t vzo_bvh_wide_depth(const vzo_bvh* b) { return b->n > 1 ? b->wdepth : 1; }
+uint32_t vzo_bvh_builder(const vzo_bvh* b) { return b->builder; }
 
 /* Wide-tree invariants: node 0 is the root; every leaf and every other wide
  * node is one slot's child exactly once; slots are filled from 0 (2 to 4 of
@@ -860,16 +1008,47 @@ static int validate_wide(const vzo_bvh* b, const vzo_sphere* s, uint32_t w, uint
     return 0;
 }
 
+/* Largest and smallest sphere centre per axis over leaves [f, l]. */
+static void leaf_centre_bounds(const vzo_bvh* b, const vzo_sphere* s, uint32_t f, uint32_t l, float lo[3],
+                               float hi[3])
+{
+    for (int k = 0; k < 3; k++) {
+        lo[k] = 3.4e38f;
+        hi[k] = -3.4e38f;
+    }
+    for (uint32_t i = f; i <= l; i++)
+        for (int k = 0; k < 3; k++) {
+            const float v = sah_cen(&s[b->leaf_prim[i]], k);
+            lo[k] = fminf(lo[k], v);
+            hi[k] = fmaxf(hi[k], v);
+        }
+}
+
 /* Structural invariants: each leaf and each non-root internal node is
  * referenced exactly once; parent boxes contain child boxes; dequantised
- * child boxes contain the exact child boxes; Morton order is sorted and
- * stable; each split is where the node's common prefix ends. */
+ * child boxes contain the exact child boxes.  LBVH: Morton order is sorted
+ * and stable; each split is where the node's common prefix ends.  SAH: the
+ * leaf order is a permutation; each node's index is an end of its leaf range
+ * (the Karras layout); above the depth cap, every split separates the two
+ * sides' sphere centres strictly along some axis, or the range's centres
+ * all coincide. */
 int vzo_bvh_validate(const vzo_bvh* b, const vzo_sphere* s)
 {
     uint32_t n = b->n;
-    for (uint32_t i = 1; i < n; i++) {
-        if (b->morton[i - 1] > b->morton[i]) return 1;
-        if (b->morton[i - 1] == b->morton[i] && b->leaf_prim[i - 1] > b->leaf_prim[i]) return 2;
+    if (b->builder == VZO_BUILD_LBVH) {
+        for (uint32_t i = 1; i < n; i++) {
+            if (b->morton[i - 1] > b->morton[i]) return 1;
+            if (b->morton[i - 1] == b->morton[i] && b->leaf_prim[i - 1] > b->leaf_prim[i]) return 2;
+        }
+    } else {
+        uint8_t* seen = (uint8_t*)calloc(n, 1);
+        int bad = 0;
+        for (uint32_t i = 0; i < n && !bad; i++) {
+            if (b->leaf_prim[i] >= n || seen[b->leaf_prim[i]]) bad = 1;
+            else seen[b->leaf_prim[i]] = 1;
+        }
+        free(seen);
+        if (bad) return 14;
     }
     if (n == 1) return 0;
     uint32_t ni = n - 1;
@@ -906,31 +1085,47 @@ int vzo_bvh_validate(const vzo_bvh* b, const vzo_sphere* s)
         if (ref_int[i] != 1) rc = 8;
     /* split property via leaf ranges */
     if (!rc) {
-        uint32_t stk[256], fst[256], lst[256];
+        uint32_t stk[256], fst[256], lst[256], dep[256];
         int sp = 0;
         stk[0] = 0;
         fst[0] = 0;
         lst[0] = n - 1;
+        dep[0] = 0;
         sp = 1;
         int nn = (int)n;
         while (sp > 0 && !rc) {
             sp--;
-            uint32_t nd = stk[sp], f = fst[sp], l = lst[sp];
+            uint32_t nd = stk[sp], f = fst[sp], l = lst[sp], d = dep[sp];
             const vzo_node* N = &b->nodes[nd];
             uint32_t g = N->child;
             if (!(g >= f && g < l)) { rc = 9; break; }
-            int dn = kdelta(b->morton, nn, (int)f, (int)l);
-            if (kdelta(b->morton, nn, (int)g, (int)g + 1) != dn) { rc = 10; break; }
+            if (b->builder == VZO_BUILD_LBVH) {
+                int dn = kdelta(b->morton, nn, (int)f, (int)l);
+                if (kdelta(b->morton, nn, (int)g, (int)g + 1) != dn) { rc = 10; break; }
+            } else {
+                if (nd != f && nd != l) { rc = 15; break; }
+                if (d < VZO_SAH_DEPTH_CAP) {
+                    float llo[3], lhi[3], rlo[3], rhi[3];
+                    leaf_centre_bounds(b, s, f, g, llo, lhi);
+                    leaf_centre_bounds(b, s, g + 1, l, rlo, rhi);
+                    int sep = 0, same = 1;
+                    for (int k = 0; k < 3; k++) {
+                        if (lhi[k] < rlo[k]) sep = 1;
+                        if (!(llo[k] == lhi[k] && rlo[k] == rhi[k] && lhi[k] == rlo[k])) same = 0;
+                    }
+                    if (!sep && !same) { rc = 16; break; }
+                }
+            }
             uint32_t flags = N->meta >> 24;
             if (((flags & 1u) != 0) != (g == f)) { rc = 11; break; }
             if (((flags & 2u) != 0) != (g + 1 == l)) { rc = 12; break; }
             if (!(flags & 1u)) {
                 if (sp >= 255) { rc = 13; break; }
-                stk[sp] = g; fst[sp] = f; lst[sp] = g; sp++;
+                stk[sp] = g; fst[sp] = f; lst[sp] = g; dep[sp] = d + 1; sp++;
             }
             if (!(flags & 2u)) {
                 if (sp >= 255) { rc = 13; break; }
-                stk[sp] = g + 1; fst[sp] = g + 1; lst[sp] = l; sp++;
+                stk[sp] = g + 1; fst[sp] = g + 1; lst[sp] = l; dep[sp] = d + 1; sp++;
             }
         }
     }

@@ -15,9 +15,11 @@
  *   accumulate .... Core/RayTracer.cu:203-216 and make_color 16-47
  *   scene ......... Core/Scene.h:13-112 (explicit x->y->z draw order)
  *
- * and the build's own LBVH (Morton codes, stable sort, Karras 2012 linking,
- * 32-byte quantised AABB-pair nodes) plus its short-stack traversal, which
- * replace OptiX's closed acceleration structure.  The GPU library must match
+ * and the build's own BVH (for more than 1024 spheres an LBVH: Morton codes,
+ * stable sort, Karras 2012 linking; for 2 to 1024 a binned-SAH tree in the
+ * same node layout; 32-byte quantised AABB-pair nodes, then the 4-wide tree
+ * collapsed from them) plus its short-stack traversal, which replace
+ * OptiX's closed acceleration structure.  The GPU library must match
  * this code bit for bit (BVH arrays, hit records, segment counts) and within
  * RMSE 1e-3 on images.  An FP64 variant of the integrator is the accuracy
  * oracle.
@@ -90,13 +92,17 @@ uint32_t vzo_bvh_num_prims(const vzo_bvh* b);
 uint32_t vzo_bvh_num_nodes(const vzo_bvh* b);
 const vzo_node* vzo_bvh_nodes(const vzo_bvh* b);
 const uint32_t* vzo_bvh_leaf_prims(const vzo_bvh* b);    /* leaf -> primitive id */
-const uint32_t* vzo_bvh_morton(const vzo_bvh* b);        /* sorted Morton codes */
+const uint32_t* vzo_bvh_morton(const vzo_bvh* b);        /* Morton key of each leaf's sphere (sorted for an LBVH) */
 const float* vzo_bvh_node_boxes(const vzo_bvh* b);       /* internal boxes, 6 floats each */
 uint32_t vzo_bvh_depth(const vzo_bvh* b);
 /* The 4-wide tree collapsed from the binary one (root = wide node 0). */
 uint32_t vzo_bvh_num_wide_nodes(const vzo_bvh* b);
 const vzo_wnode* vzo_bvh_wide_nodes(const vzo_bvh* b);
 uint32_t vzo_bvh_wide_depth(const vzo_bvh* b);
+/* Which build made the binary tree: an LBVH, or (2 to 1024 spheres) the
+ * binned-SAH build. */
+enum { VZO_BUILD_LBVH = 0, VZO_BUILD_SAH = 1 };
+uint32_t vzo_bvh_builder(const vzo_bvh* b);
 /* One wide node's slot boxes against a ray (float): masks[0]/tn_fast by the
  * traversal's one-fma-per-plane test (planes moved out by a rounding bound),
  * masks[1]/tn_exact by the exact-form robust slab test on the dequantised

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from util import (GpuTarget, book_lookat, box_lookat, gpu_params, oracle_params, rmse, sha)
+from util import (GpuTarget, book_lookat, box_lookat, geometric_scene, gpu_params, oracle_params, rmse, sha)
 from venusaur_amd import api
 from venusaur_amd.hipmem import DeviceBuffer
 
@@ -38,6 +38,7 @@ def compare_bvh(scene, spheres):
     wide = scene.download_wide()
     assert wide.tobytes() == ob.wide_nodes().tobytes()
     assert scene.info().num_wide_nodes == len(wide) and scene.info().wide_depth == ob.wide_depth()
+    assert scene.info().builder == ob.builder()
     return ob
 
 
@@ -53,12 +54,30 @@ def test_lbvh_book_bit_exact(ctx, book):
     compare_bvh(sc, s)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 2047, 2048, 2049, 4095, 4097, 20000, 300000, 1048583])
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 1024, 1025, 2047, 2048, 2049, 4095, 4097, 20000, 300000,
+                               1048583])
 def test_lbvh_sizes_bit_exact(ctx, n):
-    say(f"case: lbvh box scene n={n}")
+    """2 to 1024 spheres take the binned-SAH build, the others the LBVH."""
+    say(f"case: bvh box scene n={n}")
     s, m = api.box_scene(n - 1, 1234 + n) if n > 1 else api.box_scene(0, 1)
     sc = api.Scene(ctx, s, m)
-    compare_bvh(sc, s)
+    ob = compare_bvh(sc, s)
+    assert ob.builder() == (O.BUILD_SAH if 2 <= n <= O.SAH_MAX_PRIMS else O.BUILD_LBVH)
+
+
+@pytest.mark.parametrize("n", [200, 1024])
+def test_sah_depth_cap_bit_exact(ctx, n):
+    """Centres at x = 2^-min(i, 125): every SAH split peels off a few spheres,
+    so the build reaches its depth cap and finishes with median splits."""
+    say(f"case: sah depth cap n={n}")
+    s, m = geometric_scene(n)
+    sc = api.Scene(ctx, s, m)
+    ob = compare_bvh(sc, s)
+    assert ob.depth() > 25
+    rays = O.make_rayset(s, m, O.make_params(64, 32, 1, 4, O.camera_lookat([0.5, 0.5, -3.0], [0.5, 0.5, 0.0],
+                                                                           40.0, 2.0, 0.0, 10.0)), 2000, 2000, n)
+    got, _ = _trace(sc, rays)
+    assert got.tobytes() == ob.trace(rays)[0].tobytes()
 
 
 def test_lbvh_duplicate_centres(ctx):

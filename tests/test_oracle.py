@@ -293,6 +293,53 @@ def test_bvh_duplicates_and_degenerate_extent():
     assert O.BVH(flat).validate() == 0
 
 
+def _geometric_scene(n):
+    s, m = O.box_scene(n - 1, 3)
+    x = np.float32(2.0) ** -np.minimum(np.arange(n), 125).astype(np.float32)
+    s["cx"], s["cy"], s["cz"] = x, np.float32(0.5), np.float32(0.5)
+    s["r"] = np.maximum(x * np.float32(0.01), np.float32(1e-35))
+    return s, m
+
+
+@pytest.mark.parametrize("n,kind", [(1, O.BUILD_LBVH), (2, O.BUILD_SAH), (500, O.BUILD_SAH),
+                                    (O.SAH_MAX_PRIMS, O.BUILD_SAH), (O.SAH_MAX_PRIMS + 1, O.BUILD_LBVH)])
+def test_builder_by_size(n, kind):
+    s, m = O.box_scene(n - 1, 40 + n) if n > 1 else O.box_scene(0, 1)
+    b = O.BVH(s)
+    assert b.builder() == kind and b.validate() == 0
+    if kind == O.BUILD_LBVH and n > 1:
+        assert np.all(np.diff(b.morton().astype(np.int64)) >= 0)
+
+
+@pytest.mark.parametrize("n", [200, 1024])
+def test_sah_depth_cap(n):
+    """Centres at x = 2^-min(i, 125): each binned split peels off a few, so the
+    build reaches its depth cap (24) and ends with median splits; the tree
+    stays within 24 + 10 internal levels and 35 wide levels, and its hits
+    equal brute force."""
+    s, m = _geometric_scene(n)
+    b = O.BVH(s)
+    assert b.builder() == O.BUILD_SAH and b.validate() == 0
+    assert 24 + 1 < b.depth() <= 24 + 10 + 1 and b.wide_depth() <= 35
+    cam = O.camera_lookat([0.5, 0.5, -3.0], [0.5, 0.5, 0.0], 40.0, 2.0, 0.0, 10.0)
+    rays = O.make_rayset(s, m, O.make_params(64, 32, 1, 4, cam), 2000, 2000, n)
+    assert b.trace(rays)[0].tobytes() == O.trace_brute(s, rays).tobytes()
+
+
+def test_sah_tree_on_the_book_scene(book):
+    """The reason small scenes get the SAH build: the book scene's paths
+    visit fewer than 6.5 wide nodes per segment (its Morton-order LBVH took
+    7.3 on the 1080p frame), and the frame equals the brute-force render bit
+    for bit."""
+    s, m = book
+    W, H = 320, 180
+    p = O.make_params(W, H, 2, 50, O.camera_lookat([13, 2, 3], [0, 0, 0], 20.0, float(np.float32(W) / np.float32(H)),
+                                                   0.1, 10.0))
+    acc_s, _, st_s = O.render(s, m, p, bvh=O.BVH(s), want_image=False)
+    assert st_s.node_visits / st_s.segments < 6.5
+    assert acc_s.tobytes() == O.render(s, m, p, bvh=None, want_image=False)[0].tobytes()
+
+
 def test_bvh_fixture_hashes(book):
     import hashlib
     b = O.BVH(book[0])

@@ -57,6 +57,19 @@ class GpuTarget:
         return self.image.download(np.uint8, (self.h, self.w, 4))
 
 
+def geometric_scene(n):
+    """n spheres with centres at x = 2^-min(i, 125) (y = z = 0.5) and radii
+    max(x / 100, 1e-35), all normal floats: a binned-SAH split can only peel
+    a few of them off at a time, and the last ones share one centre."""
+    s, m = O.box_scene(n - 1, 3)
+    x = np.float32(2.0) ** -np.minimum(np.arange(n), 125).astype(np.float32)
+    s["cx"] = x
+    s["cy"] = np.float32(0.5)
+    s["cz"] = np.float32(0.5)
+    s["r"] = np.maximum(x * np.float32(0.01), np.float32(1e-35))
+    return s, m
+
+
 def rmse(a, b):
     d = a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)
     return float(np.sqrt(np.mean(d * d)))

@@ -61,7 +61,7 @@ class Stats(C.Structure):
 class AccelInfo(C.Structure):
     _fields_ = [("num_prims", C.c_uint32), ("num_nodes", C.c_uint32), ("depth", C.c_uint32),
                 ("node_bytes", C.c_uint32), ("device_bytes", C.c_uint64), ("build_ms", C.c_double),
-                ("num_wide_nodes", C.c_uint32), ("wide_depth", C.c_uint32)]
+                ("num_wide_nodes", C.c_uint32), ("wide_depth", C.c_uint32), ("builder", C.c_uint32)]
 
 
 class BatchItem(C.Structure):

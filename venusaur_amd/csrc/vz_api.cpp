@@ -327,6 +327,7 @@ vzStatus vzAccelInfoGet(vzScene scene, vzAccelInfo* out)
     out->build_ms = scene->build_ms;
     out->num_wide_nodes = scene->nw;
     out->wide_depth = scene->wdepth;
+    out->builder = scene->builder;
     return VZ_SUCCESS;
 }
 

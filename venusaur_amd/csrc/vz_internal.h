@@ -152,6 +152,7 @@ struct vzScene_st {
     uint32_t* d_wcnt = nullptr;       // per level frontier counts, then nw and the wide depth
     uint32_t nw = 0;                  // wide nodes
     uint32_t wdepth = 0;              // wide levels from the root to the deepest leaf
+    uint32_t builder = 0;             // VZ_BUILD_LBVH or VZ_BUILD_SAH
     uint32_t depth = 0;
     uint64_t device_bytes = 0;
     double build_ms = 0.0;

@@ -19,6 +19,10 @@
 //   5. Karras linking      one thread per internal node
 //   6. refit + pack        one thread per leaf walks up; the second arrival at
 //                          a node computes its box and writes the 32-byte node
+//   7. 4-wide collapse     level by level, then a structural check
+// A scene of 2 to kSahMaxPrims spheres replaces stages 3 and 5 by a
+// binned-SAH build in one workgroup (k_sah_build) that writes the same node
+// layout, so stages 4, 6 and 7 are shared.
 #include "vz_device.h"
 #include "vz_internal.h"
 
@@ -363,6 +367,247 @@ __global__ void __launch_bounds__(kBlk) k_karras(const uint32_t* __restrict__ K,
     if (mn == g) { f |= 1u; parent_leaf[g] = up; } else { parent_int[g] = up; }
     if (mx == g + 1) { f |= 2u; parent_leaf[g + 1] = up; } else { parent_int[g + 1] = up; }
     child[i] = (uint32_t)g | (f << 30);
+}
+
+// ---- 5'. binned-SAH build of a small scene -------------------------------------------
+// For 2 .. kSahMaxPrims spheres the Morton order serves large spheres badly
+// (the book scene's ground and its three big spheres), so the binary tree is
+// built top down by the surface-area heuristic instead.  A range [a, b] of
+// the leaf permutation at depth < kSahDepthCap: per axis whose centre extent
+// ext is positive, sphere centre c falls in bin min(31, (int)(((c - lo) /
+// ext) * 32)); the split after bin j costs area(L) * nL + area(R) * nR over
+// the leaf-box unions and counts of bins <= j and > j (both non-empty); the
+// cheapest wins, lower axis then lower bin on a tie, and the range is
+// partitioned stably.  With no positive extent, or below the depth cap, the
+// range splits after its first floor(m / 2) spheres.  The cap bounds the
+// tree at kSahDepthCap + 10 internal levels for 1024 spheres, so the 4-wide
+// tree stays within the traversal's kMaxWideDepth.
+// Nodes use the Karras layout (children g and g + 1 for a split after leaf
+// position g, the root is node 0), so refit, collapse and traversal are the
+// LBVH's.  One workgroup, level by level: each wave takes one range of the
+// level per round.  Bin boxes are kept as order-preserving integer keys and
+// merged with LDS integer min/max, counts with adds, so the bins do not
+// depend on the order lanes ran in; the sweep and the partition are
+// sequential in bin and leaf order.  The result equals the oracle's
+// sah_range bit for bit.
+constexpr uint32_t kSahMaxPrims = 1024;
+constexpr int kSahBins = 32;
+constexpr uint32_t kSahDepthCap = 24;
+constexpr int kSahWaves = 8;
+constexpr int kSahThreads = 64 * kSahWaves;
+constexpr int kSahBinWords = 3 * kSahBins * 7;  // per wave: [axis][bin] lo xyz, hi xyz keys, count
+static_assert(kSahMaxPrims <= kRefitLeaves, "every SAH range lies in one refit workgroup");
+static_assert(kSahDepthCap + 10 + 1 <= kMaxWideDepth, "SAH tree depth within the traversal stack");
+
+__device__ __forceinline__ uint32_t ord_key(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_val(uint32_t k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+__device__ __forceinline__ float sah_cen(const float4 s, int k) { return k == 0 ? s.x : (k == 1 ? s.y : s.z); }
+__device__ __forceinline__ int sah_bin(float c, float lo, float ext)
+{
+    const int j = (int)(((c - lo) / ext) * (float)kSahBins);
+    return j > kSahBins - 1 ? kSahBins - 1 : j;
+}
+__device__ __forceinline__ float box_area6(const float* b)
+{
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ void __launch_bounds__(kSahThreads) k_sah_build(const float4* __restrict__ sph, uint32_t n,
+                                                           const uint32_t* __restrict__ keys_in,
+                                                           uint32_t* __restrict__ leaf_prim,
+                                                           uint32_t* __restrict__ keys_out,
+                                                           uint32_t* __restrict__ child,
+                                                           uint32_t* __restrict__ parent_int,
+                                                           uint32_t* __restrict__ parent_leaf)
+{
+    __shared__ uint32_t s_perm[kSahMaxPrims], s_tmp[kSahMaxPrims];
+    __shared__ uint32_t s_task[2][kSahMaxPrims / 2];  // a | b << 10 | self << 20
+    __shared__ uint32_t s_ntask[2];
+    __shared__ uint32_t s_bin[kSahWaves][kSahBinWords];
+    __shared__ float s_ra[kSahWaves][3][kSahBins];
+    __shared__ uint32_t s_rn[kSahWaves][3][kSahBins];
+    __shared__ float s_cost[kSahWaves][3];
+    __shared__ int s_cbin[kSahWaves][3];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    for (uint32_t i = tid; i < n; i += kSahThreads) s_perm[i] = i;
+    if (tid == 0) {
+        s_task[0][0] = (n - 1u) << 10;
+        s_ntask[0] = 1u;
+        s_ntask[1] = 0u;
+        parent_int[0] = 0xFFFFFFFFu;
+    }
+    uint32_t cur = 0;
+    for (uint32_t level = 0; level < 64u; level++) {
+        __syncthreads();
+        const uint32_t nt = s_ntask[cur];
+        if (nt == 0u) break;
+        for (uint32_t round = 0; round < nt; round += kSahWaves) {
+            const uint32_t t = round + w;
+            const bool has = t < nt;
+            const uint32_t tw = has ? s_task[cur][t] : 0u;
+            const uint32_t a = tw & 0x3FFu, b = (tw >> 10) & 0x3FFu, self = tw >> 20;
+            // centre bounds of the range
+            float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+            if (has)
+                for (uint32_t i = a + lane; i <= b; i += 64u) {
+                    const float4 sp = sph[s_perm[i]];
+                    for (int k = 0; k < 3; k++) {
+                        lo[k] = fminf(lo[k], sah_cen(sp, k));
+                        hi[k] = fmaxf(hi[k], sah_cen(sp, k));
+                    }
+                }
+            float ext[3];
+            bool any = false;
+            for (int k = 0; k < 3; k++) {
+                lo[k] = wave_min(lo[k]);
+                hi[k] = wave_max(hi[k]);
+                ext[k] = hi[k] - lo[k];
+                any = any || ext[k] > 0.0f;
+            }
+            const bool sah = has && level < kSahDepthCap && any;
+            if (sah)
+                for (int x = (int)lane; x < kSahBinWords; x += 64) {
+                    const int q = x % 7;
+                    s_bin[w][x] = q < 3 ? 0xFFFFFFFFu : 0u;
+                }
+            __syncthreads();
+            if (sah)
+                for (uint32_t i = a + lane; i <= b; i += 64u) {
+                    const float4 sp = sph[s_perm[i]];
+                    const float r = fabsf(sp.w);
+                    const float lb[6] = {sp.x - r, sp.y - r, sp.z - r, sp.x + r, sp.y + r, sp.z + r};
+                    for (int k = 0; k < 3; k++) {
+                        if (!(ext[k] > 0.0f)) continue;
+                        uint32_t* bn = &s_bin[w][(k * kSahBins + sah_bin(sah_cen(sp, k), lo[k], ext[k])) * 7];
+                        for (int q = 0; q < 3; q++) {
+                            atomicMin(&bn[q], ord_key(lb[q]));
+                            atomicMax(&bn[3 + q], ord_key(lb[3 + q]));
+                        }
+                        atomicAdd(&bn[6], 1u);
+                    }
+                }
+            __syncthreads();
+            // one lane per axis: suffix areas and counts, then the prefix sweep
+            if (lane < 3u) {
+                const int k = (int)lane;
+                int best = -1;
+                float bcost = 0.0f;
+                if (sah && ext[k] > 0.0f) {
+                    const uint32_t* bins = &s_bin[w][k * kSahBins * 7];
+                    float R[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
+                    uint32_t nr = 0;
+                    for (int j = kSahBins - 1; j >= 1; j--) {
+                        const uint32_t c = bins[j * 7 + 6];
+                        if (c) {
+                            for (int q = 0; q < 3; q++) {
+                                R[q] = fminf(R[q], ord_val(bins[j * 7 + q]));
+                                R[3 + q] = fmaxf(R[3 + q], ord_val(bins[j * 7 + 3 + q]));
+                            }
+                        }
+                        nr += c;
+                        s_ra[w][k][j] = box_area6(R);
+                        s_rn[w][k][j] = nr;
+                    }
+                    float L[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
+                    uint32_t nl = 0;
+                    for (int j = 0; j < kSahBins - 1; j++) {
+                        const uint32_t c = bins[j * 7 + 6];
+                        if (c) {
+                            for (int q = 0; q < 3; q++) {
+                                L[q] = fminf(L[q], ord_val(bins[j * 7 + q]));
+                                L[3 + q] = fmaxf(L[3 + q], ord_val(bins[j * 7 + 3 + q]));
+                            }
+                        }
+                        nl += c;
+                        const uint32_t rn = s_rn[w][k][j + 1];
+                        if (nl == 0u || rn == 0u) continue;
+                        const float cost = box_area6(L) * (float)nl + s_ra[w][k][j + 1] * (float)rn;
+                        if (best < 0 || cost < bcost) {
+                            best = j;
+                            bcost = cost;
+                        }
+                    }
+                }
+                s_cost[w][k] = bcost;
+                s_cbin[w][k] = best;
+            }
+            __syncthreads();
+            int bk = -1, bj = -1;
+            float bc = 0.0f;
+            if (sah)
+                for (int k = 0; k < 3; k++)
+                    if (s_cbin[w][k] >= 0 && (bk < 0 || s_cost[w][k] < bc)) {
+                        bk = k;
+                        bj = s_cbin[w][k];
+                        bc = s_cost[w][k];
+                    }
+            uint32_t g = a + (b - a + 1u) / 2u - 1u;
+            if (bk >= 0) {
+                uint32_t cl = 0;
+                for (uint32_t i = a + lane; i <= b; i += 64u)
+                    cl += sah_bin(sah_cen(sph[s_perm[i]], bk), lo[bk], ext[bk]) <= bj ? 1u : 0u;
+                const uint32_t nl = wave_sum(cl);
+                uint32_t offl = a, offr = a + nl;
+                for (uint32_t i0 = a; i0 <= b; i0 += 64u) {
+                    const uint32_t i = i0 + lane;
+                    const bool valid = i <= b;
+                    const uint32_t p = valid ? s_perm[i] : 0u;
+                    const bool left = valid && sah_bin(sah_cen(sph[p], bk), lo[bk], ext[bk]) <= bj;
+                    const uint64_t ml = __ballot(left), mr = __ballot(valid && !left);
+                    if (left) s_tmp[offl + mbcnt(ml)] = p;
+                    else if (valid) s_tmp[offr + mbcnt(mr)] = p;
+                    offl += (uint32_t)__popcll(ml);
+                    offr += (uint32_t)__popcll(mr);
+                }
+                g = a + nl - 1u;
+            }
+            __syncthreads();
+            if (bk >= 0)
+                for (uint32_t i = a + lane; i <= b; i += 64u) s_perm[i] = s_tmp[i];
+            if (has && lane == 0u) {
+                const uint32_t up = self | ((a / kRefitLeaves == b / kRefitLeaves) ? kLocalBit : 0u);
+                uint32_t f = 0;
+                if (g == a) { f |= 1u; parent_leaf[g] = up; } else { parent_int[g] = up; }
+                if (g + 1u == b) { f |= 2u; parent_leaf[g + 1u] = up; } else { parent_int[g + 1u] = up; }
+                child[self] = g | (f << 30);
+                if (g > a) s_task[cur ^ 1u][atomicAdd(&s_ntask[cur ^ 1u], 1u)] = a | (g << 10) | (g << 20);
+                if (g + 1u < b)
+                    s_task[cur ^ 1u][atomicAdd(&s_ntask[cur ^ 1u], 1u)] = (g + 1u) | (b << 10) | ((g + 1u) << 20);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) s_ntask[cur] = 0u;
+        cur ^= 1u;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kSahThreads) {
+        const uint32_t p = s_perm[i];
+        leaf_prim[i] = p;
+        keys_out[i] = keys_in[p];
+    }
 }
 
 // ---- 6. refit + pack ---------------------------------------------------------------
@@ -957,21 +1202,30 @@ vzStatus lbvh_build(vzScene s, hipStream_t st)
     hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlk), 0, st, partial, nb, 1u, grid_b);
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in, n, fin, grid_b,
                        s->d_keys[0], s->d_vals[0]);
-    uint32_t tiles = s->hist_blocks;
-    uint32_t* dtot = s->d_hist + 256 * tiles;
-    for (int pass = 0; pass < 4; pass++) {
-        int src = pass & 1, dst = src ^ 1;
-        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src], n, pass * 8,
-                           tiles, s->d_hist);
-        hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(kBlk), 0, st, s->d_hist, tiles, dtot);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src],
-                           s->d_vals[src], n, pass * 8, tiles, s->d_hist, dtot, s->d_keys[dst], s->d_vals[dst]);
+    s->builder = (n >= 2 && n <= kSahMaxPrims) ? VZ_BUILD_SAH : VZ_BUILD_LBVH;
+    if (s->builder == VZ_BUILD_SAH) {
+        // keys stay in sphere order in buffer 0; leaf order and its keys land in buffers 1
+        hipLaunchKernelGGL(k_sah_build, dim3(1), dim3(kSahThreads), 0, st, s->d_sph_in, n, s->d_keys[0],
+                           s->d_vals[1], s->d_keys[1], s->d_child, s->d_parent_int, s->d_parent_leaf);
+        s->d_morton = s->d_keys[1];
+        s->d_leaf_prim = s->d_vals[1];
+    } else {
+        uint32_t tiles = s->hist_blocks;
+        uint32_t* dtot = s->d_hist + 256 * tiles;
+        for (int pass = 0; pass < 4; pass++) {
+            int src = pass & 1, dst = src ^ 1;
+            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src], n, pass * 8,
+                               tiles, s->d_hist);
+            hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(kBlk), 0, st, s->d_hist, tiles, dtot);
+            hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src],
+                               s->d_vals[src], n, pass * 8, tiles, s->d_hist, dtot, s->d_keys[dst], s->d_vals[dst]);
+        }
+        s->d_morton = s->d_keys[0];  // four passes: result back in buffer 0
+        s->d_leaf_prim = s->d_vals[0];
     }
-    s->d_morton = s->d_keys[0];  // four passes: result back in buffer 0
-    s->d_leaf_prim = s->d_vals[0];
     hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in,
                        s->d_mat_in, s->d_leaf_prim, n, s->d_sph, s->d_leaf_info, s->d_mat);
-    if (n >= 2) {
+    if (n >= 2 && s->builder == VZ_BUILD_LBVH) {
         hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, kBlk)), dim3(kBlk), 0, st, s->d_morton, n,
                            s->d_child, s->d_parent_int, s->d_parent_leaf, s->d_visit);
     }
