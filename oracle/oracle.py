@@ -24,6 +24,7 @@ WNODE_DT = np.dtype([("o", "<f4", 3), ("meta", "<u4"), ("qlo", "<u4", 3), ("qhi"
 MISS = 0xFFFFFFFF
 BUILD_LBVH, BUILD_SAH = 0, 1
 SAH_MAX_PRIMS = 1024
+TREELET_MIN_PRIMS = 131072
 
 
 class Camera(C.Structure):

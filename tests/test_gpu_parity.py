@@ -54,10 +54,11 @@ def test_lbvh_book_bit_exact(ctx, book):
     compare_bvh(sc, s)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 1024, 1025, 2047, 2048, 2049, 4095, 4097, 20000, 300000,
-                               1048583])
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 1024, 1025, 2047, 2048, 2049, 4095, 4097, 20000, 131071,
+                               131072, 300000, 1048583])
 def test_lbvh_sizes_bit_exact(ctx, n):
-    """2 to 1024 spheres take the binned-SAH build, the others the LBVH."""
+    """2 to 1024 spheres take the binned-SAH build, the others the LBVH, with
+    SAH treelets from 131072 spheres up."""
     say(f"case: bvh box scene n={n}")
     s, m = api.box_scene(n - 1, 1234 + n) if n > 1 else api.box_scene(0, 1)
     sc = api.Scene(ctx, s, m)

@@ -326,6 +326,22 @@ def test_sah_depth_cap(n):
     assert b.trace(rays)[0].tobytes() == O.trace_brute(s, rays).tobytes()
 
 
+@pytest.mark.parametrize("n", [O.TREELET_MIN_PRIMS - 1, O.TREELET_MIN_PRIMS])
+def test_lbvh_treelets(n):
+    """From TREELET_MIN_PRIMS spheres up the LBVH's treelets are SAH trees:
+    the validator sorts each back to Morton order and checks the LBVH above
+    and the SAH layout inside; hits equal brute force either way."""
+    s, m = O.box_scene(n - 1, 11)
+    b = O.BVH(s)
+    assert b.builder() == O.BUILD_LBVH and b.validate() == 0
+    keys = b.morton().astype(np.int64)
+    assert (np.all(np.diff(keys) >= 0)) == (n < O.TREELET_MIN_PRIMS)
+    assert np.array_equal(np.sort(b.leaf_prims()), np.arange(n))
+    cam = O.camera_lookat([0, 15, -120], [0, 0, 0], 40.0, 2.0, 0.0, 10.0)
+    rays = O.make_rayset(s, m, O.make_params(64, 32, 1, 8, cam), 300, 300, n)
+    assert b.trace(rays)[0].tobytes() == O.trace_brute(s, rays).tobytes()
+
+
 def test_sah_tree_on_the_book_scene(book):
     """The reason small scenes get the SAH build: the book scene's paths
     visit fewer than 6.5 wide nodes per segment (its Morton-order LBVH took

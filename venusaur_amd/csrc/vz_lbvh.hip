@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(kBlk) k_karras(const uint32_t* __restrict__ K,
                                                  uint32_t* __restrict__ child,
                                                  uint32_t* __restrict__ parent_int,
                                                  uint32_t* __restrict__ parent_leaf,
-                                                 uint32_t* __restrict__ visit)
+                                                 uint32_t* __restrict__ visit, uint2* __restrict__ range)
 {
     int n = (int)n_;
     int i = (int)(blockIdx.x * kBlk + threadIdx.x);
@@ -367,6 +367,7 @@ __global__ void __launch_bounds__(kBlk) k_karras(const uint32_t* __restrict__ K,
     if (mn == g) { f |= 1u; parent_leaf[g] = up; } else { parent_int[g] = up; }
     if (mx == g + 1) { f |= 2u; parent_leaf[g + 1] = up; } else { parent_int[g + 1] = up; }
     child[i] = (uint32_t)g | (f << 30);
+    range[i] = make_uint2((uint32_t)mn, (uint32_t)mx);
 }
 
 // ---- 5'. binned-SAH build of a small scene -------------------------------------------
@@ -393,6 +394,14 @@ __global__ void __launch_bounds__(kBlk) k_karras(const uint32_t* __restrict__ K,
 constexpr uint32_t kSahMaxPrims = 1024;
 constexpr int kSahBins = 32;
 constexpr uint32_t kSahDepthCap = 24;
+// An LBVH of at least kTreeletMinPrims spheres has its treelets rebuilt by
+// the same SAH build: every maximal subtree of at most kSahMaxPrims leaves
+// whose root lies above kSahDepthCap, one workgroup each, splits counting
+// depth from the whole tree's root.  On box scenes of a few thousand
+// spheres SAH treelets visited more nodes than Morton order; from about 2e5
+// spheres up they visit 4-6% fewer.
+constexpr uint32_t kTreeletMinPrims = 131072;
+constexpr uint32_t kTreeletBlocks = 2048;  // grid of k_sah_treelets (grid-stride over the treelets)
 constexpr int kSahWaves = 8;
 constexpr int kSahThreads = 64 * kSahWaves;
 constexpr int kSahBinWords = 3 * kSahBins * 7;  // per wave: [axis][bin] lo xyz, hi xyz keys, count
@@ -435,45 +444,59 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
-__global__ void __launch_bounds__(kSahThreads) k_sah_build(const float4* __restrict__ sph, uint32_t n,
-                                                           const uint32_t* __restrict__ keys_in,
-                                                           uint32_t* __restrict__ leaf_prim,
-                                                           uint32_t* __restrict__ keys_out,
-                                                           uint32_t* __restrict__ child,
-                                                           uint32_t* __restrict__ parent_int,
-                                                           uint32_t* __restrict__ parent_leaf)
+// LDS of one treelet build: the range's primitives and keys by local
+// index, the local permutation, two level lists of ranges, and per wave the
+// bins and the sweep's suffix areas and counts.
+struct SahShared {
+    uint32_t prim[kSahMaxPrims], key[kSahMaxPrims];
+    uint32_t perm[kSahMaxPrims], tmp[kSahMaxPrims];
+    uint2 task[2][kSahMaxPrims / 2];  // (a | b << 10 local, node index)
+    uint32_t ntask[2];
+    uint32_t bin[kSahWaves][kSahBinWords];
+    float ra[kSahWaves][3][kSahBins];
+    uint32_t rn[kSahWaves][3][kSahBins];
+    float cost[kSahWaves][3];
+    int cbin[kSahWaves][3];
+};
+
+// Rebuilds the subtree over leaves [f, f + m) (2 <= m <= kSahMaxPrims), root
+// node r at depth d0, in place: leaf_prim and keys are permuted within the
+// range, and the child words and parent links of the subtree's nodes are
+// rewritten.  In the Karras layout a subtree's node indices are fixed by its
+// leaf range and root index, so the nodes above it are untouched.  Called
+// by a whole workgroup.
+__device__ __forceinline__ void sah_treelet(const float4* __restrict__ sph, uint32_t f, uint32_t m, uint32_t r,
+                                            uint32_t d0, uint32_t* __restrict__ leaf_prim,
+                                            uint32_t* __restrict__ keys, uint32_t* __restrict__ child,
+                                            uint32_t* __restrict__ parent_int, uint32_t* __restrict__ parent_leaf,
+                                            SahShared& sh)
 {
-    __shared__ uint32_t s_perm[kSahMaxPrims], s_tmp[kSahMaxPrims];
-    __shared__ uint32_t s_task[2][kSahMaxPrims / 2];  // a | b << 10 | self << 20
-    __shared__ uint32_t s_ntask[2];
-    __shared__ uint32_t s_bin[kSahWaves][kSahBinWords];
-    __shared__ float s_ra[kSahWaves][3][kSahBins];
-    __shared__ uint32_t s_rn[kSahWaves][3][kSahBins];
-    __shared__ float s_cost[kSahWaves][3];
-    __shared__ int s_cbin[kSahWaves][3];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    for (uint32_t i = tid; i < n; i += kSahThreads) s_perm[i] = i;
+    for (uint32_t i = tid; i < m; i += kSahThreads) {
+        sh.prim[i] = leaf_prim[f + i];
+        sh.key[i] = keys[f + i];
+        sh.perm[i] = i;
+    }
     if (tid == 0) {
-        s_task[0][0] = (n - 1u) << 10;
-        s_ntask[0] = 1u;
-        s_ntask[1] = 0u;
-        parent_int[0] = 0xFFFFFFFFu;
+        sh.task[0][0] = make_uint2((m - 1u) << 10, r);
+        sh.ntask[0] = 1u;
+        sh.ntask[1] = 0u;
     }
     uint32_t cur = 0;
     for (uint32_t level = 0; level < 64u; level++) {
         __syncthreads();
-        const uint32_t nt = s_ntask[cur];
+        const uint32_t nt = sh.ntask[cur];
         if (nt == 0u) break;
         for (uint32_t round = 0; round < nt; round += kSahWaves) {
             const uint32_t t = round + w;
             const bool has = t < nt;
-            const uint32_t tw = has ? s_task[cur][t] : 0u;
-            const uint32_t a = tw & 0x3FFu, b = (tw >> 10) & 0x3FFu, self = tw >> 20;
+            const uint2 tw = has ? sh.task[cur][t] : make_uint2(0u, 0u);
+            const uint32_t a = tw.x & 0x3FFu, b = (tw.x >> 10) & 0x3FFu, self = tw.y;
             // centre bounds of the range
             float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
             if (has)
                 for (uint32_t i = a + lane; i <= b; i += 64u) {
-                    const float4 sp = sph[s_perm[i]];
+                    const float4 sp = sph[sh.prim[sh.perm[i]]];
                     for (int k = 0; k < 3; k++) {
                         lo[k] = fminf(lo[k], sah_cen(sp, k));
                         hi[k] = fmaxf(hi[k], sah_cen(sp, k));
@@ -487,21 +510,18 @@ __global__ void __launch_bounds__(kSahThreads) k_sah_build(const float4* __restr
                 ext[k] = hi[k] - lo[k];
                 any = any || ext[k] > 0.0f;
             }
-            const bool sah = has && level < kSahDepthCap && any;
+            const bool sah = has && d0 + level < kSahDepthCap && any;
             if (sah)
-                for (int x = (int)lane; x < kSahBinWords; x += 64) {
-                    const int q = x % 7;
-                    s_bin[w][x] = q < 3 ? 0xFFFFFFFFu : 0u;
-                }
+                for (int x = (int)lane; x < kSahBinWords; x += 64) sh.bin[w][x] = x % 7 < 3 ? 0xFFFFFFFFu : 0u;
             __syncthreads();
             if (sah)
                 for (uint32_t i = a + lane; i <= b; i += 64u) {
-                    const float4 sp = sph[s_perm[i]];
-                    const float r = fabsf(sp.w);
-                    const float lb[6] = {sp.x - r, sp.y - r, sp.z - r, sp.x + r, sp.y + r, sp.z + r};
+                    const float4 sp = sph[sh.prim[sh.perm[i]]];
+                    const float rr = fabsf(sp.w);
+                    const float lb[6] = {sp.x - rr, sp.y - rr, sp.z - rr, sp.x + rr, sp.y + rr, sp.z + rr};
                     for (int k = 0; k < 3; k++) {
                         if (!(ext[k] > 0.0f)) continue;
-                        uint32_t* bn = &s_bin[w][(k * kSahBins + sah_bin(sah_cen(sp, k), lo[k], ext[k])) * 7];
+                        uint32_t* bn = &sh.bin[w][(k * kSahBins + sah_bin(sah_cen(sp, k), lo[k], ext[k])) * 7];
                         for (int q = 0; q < 3; q++) {
                             atomicMin(&bn[q], ord_key(lb[q]));
                             atomicMax(&bn[3 + q], ord_key(lb[3 + q]));
@@ -516,7 +536,7 @@ __global__ void __launch_bounds__(kSahThreads) k_sah_build(const float4* __restr
                 int best = -1;
                 float bcost = 0.0f;
                 if (sah && ext[k] > 0.0f) {
-                    const uint32_t* bins = &s_bin[w][k * kSahBins * 7];
+                    const uint32_t* bins = &sh.bin[w][k * kSahBins * 7];
                     float R[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
                     uint32_t nr = 0;
                     for (int j = kSahBins - 1; j >= 1; j--) {
@@ -528,8 +548,8 @@ __global__ void __launch_bounds__(kSahThreads) k_sah_build(const float4* __restr
                             }
                         }
                         nr += c;
-                        s_ra[w][k][j] = box_area6(R);
-                        s_rn[w][k][j] = nr;
+                        sh.ra[w][k][j] = box_area6(R);
+                        sh.rn[w][k][j] = nr;
                     }
                     float L[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
                     uint32_t nl = 0;
@@ -542,43 +562,43 @@ __global__ void __launch_bounds__(kSahThreads) k_sah_build(const float4* __restr
                             }
                         }
                         nl += c;
-                        const uint32_t rn = s_rn[w][k][j + 1];
+                        const uint32_t rn = sh.rn[w][k][j + 1];
                         if (nl == 0u || rn == 0u) continue;
-                        const float cost = box_area6(L) * (float)nl + s_ra[w][k][j + 1] * (float)rn;
+                        const float cost = box_area6(L) * (float)nl + sh.ra[w][k][j + 1] * (float)rn;
                         if (best < 0 || cost < bcost) {
                             best = j;
                             bcost = cost;
                         }
                     }
                 }
-                s_cost[w][k] = bcost;
-                s_cbin[w][k] = best;
+                sh.cost[w][k] = bcost;
+                sh.cbin[w][k] = best;
             }
             __syncthreads();
             int bk = -1, bj = -1;
             float bc = 0.0f;
             if (sah)
                 for (int k = 0; k < 3; k++)
-                    if (s_cbin[w][k] >= 0 && (bk < 0 || s_cost[w][k] < bc)) {
+                    if (sh.cbin[w][k] >= 0 && (bk < 0 || sh.cost[w][k] < bc)) {
                         bk = k;
-                        bj = s_cbin[w][k];
-                        bc = s_cost[w][k];
+                        bj = sh.cbin[w][k];
+                        bc = sh.cost[w][k];
                     }
             uint32_t g = a + (b - a + 1u) / 2u - 1u;
             if (bk >= 0) {
                 uint32_t cl = 0;
                 for (uint32_t i = a + lane; i <= b; i += 64u)
-                    cl += sah_bin(sah_cen(sph[s_perm[i]], bk), lo[bk], ext[bk]) <= bj ? 1u : 0u;
+                    cl += sah_bin(sah_cen(sph[sh.prim[sh.perm[i]]], bk), lo[bk], ext[bk]) <= bj ? 1u : 0u;
                 const uint32_t nl = wave_sum(cl);
                 uint32_t offl = a, offr = a + nl;
                 for (uint32_t i0 = a; i0 <= b; i0 += 64u) {
                     const uint32_t i = i0 + lane;
                     const bool valid = i <= b;
-                    const uint32_t p = valid ? s_perm[i] : 0u;
-                    const bool left = valid && sah_bin(sah_cen(sph[p], bk), lo[bk], ext[bk]) <= bj;
+                    const uint32_t p = valid ? sh.perm[i] : 0u;
+                    const bool left = valid && sah_bin(sah_cen(sph[sh.prim[p]], bk), lo[bk], ext[bk]) <= bj;
                     const uint64_t ml = __ballot(left), mr = __ballot(valid && !left);
-                    if (left) s_tmp[offl + mbcnt(ml)] = p;
-                    else if (valid) s_tmp[offr + mbcnt(mr)] = p;
+                    if (left) sh.tmp[offl + mbcnt(ml)] = p;
+                    else if (valid) sh.tmp[offr + mbcnt(mr)] = p;
                     offl += (uint32_t)__popcll(ml);
                     offr += (uint32_t)__popcll(mr);
                 }
@@ -586,27 +606,85 @@ __global__ void __launch_bounds__(kSahThreads) k_sah_build(const float4* __restr
             }
             __syncthreads();
             if (bk >= 0)
-                for (uint32_t i = a + lane; i <= b; i += 64u) s_perm[i] = s_tmp[i];
+                for (uint32_t i = a + lane; i <= b; i += 64u) sh.perm[i] = sh.tmp[i];
             if (has && lane == 0u) {
-                const uint32_t up = self | ((a / kRefitLeaves == b / kRefitLeaves) ? kLocalBit : 0u);
-                uint32_t f = 0;
-                if (g == a) { f |= 1u; parent_leaf[g] = up; } else { parent_int[g] = up; }
-                if (g + 1u == b) { f |= 2u; parent_leaf[g + 1u] = up; } else { parent_int[g + 1u] = up; }
-                child[self] = g | (f << 30);
-                if (g > a) s_task[cur ^ 1u][atomicAdd(&s_ntask[cur ^ 1u], 1u)] = a | (g << 10) | (g << 20);
+                const uint32_t G = f + g;  // split after leaf G: children G and G + 1
+                const uint32_t up = self | (((f + a) / kRefitLeaves == (f + b) / kRefitLeaves) ? kLocalBit : 0u);
+                uint32_t fl = 0;
+                if (g == a) { fl |= 1u; parent_leaf[G] = up; } else { parent_int[G] = up; }
+                if (g + 1u == b) { fl |= 2u; parent_leaf[G + 1u] = up; } else { parent_int[G + 1u] = up; }
+                child[self] = G | (fl << 30);
+                if (g > a) sh.task[cur ^ 1u][atomicAdd(&sh.ntask[cur ^ 1u], 1u)] = make_uint2(a | (g << 10), G);
                 if (g + 1u < b)
-                    s_task[cur ^ 1u][atomicAdd(&s_ntask[cur ^ 1u], 1u)] = (g + 1u) | (b << 10) | ((g + 1u) << 20);
+                    sh.task[cur ^ 1u][atomicAdd(&sh.ntask[cur ^ 1u], 1u)] = make_uint2((g + 1u) | (b << 10), G + 1u);
             }
         }
         __syncthreads();
-        if (tid == 0) s_ntask[cur] = 0u;
+        if (tid == 0) sh.ntask[cur] = 0u;
         cur ^= 1u;
     }
     __syncthreads();
-    for (uint32_t i = tid; i < n; i += kSahThreads) {
-        const uint32_t p = s_perm[i];
-        leaf_prim[i] = p;
-        keys_out[i] = keys_in[p];
+    for (uint32_t i = tid; i < m; i += kSahThreads) {
+        const uint32_t p = sh.perm[i];
+        leaf_prim[f + i] = sh.prim[p];
+        keys[f + i] = sh.key[p];
+    }
+    __syncthreads();  // the LDS is reused by the next treelet
+}
+
+// A small scene: the whole tree is one SAH build (keys and identity
+// permutation in sphere order in, leaf order out).
+__global__ void __launch_bounds__(kSahThreads) k_sah_build(const float4* __restrict__ sph, uint32_t n,
+                                                           uint32_t* __restrict__ leaf_prim,
+                                                           uint32_t* __restrict__ keys,
+                                                           uint32_t* __restrict__ child,
+                                                           uint32_t* __restrict__ parent_int,
+                                                           uint32_t* __restrict__ parent_leaf)
+{
+    __shared__ SahShared sh;
+    if (threadIdx.x == 0) parent_int[0] = 0xFFFFFFFFu;
+    sah_treelet(sph, 0u, n, 0u, 0u, leaf_prim, keys, child, parent_int, parent_leaf, sh);
+}
+
+// A large LBVH: the roots of its treelets, the maximal subtrees of at most
+// kSahMaxPrims leaves whose root lies above kSahDepthCap.  Internal node i
+// (leaves range[i]) is one when it is small enough and its parent is not;
+// its depth is the length of its parent chain.  Appended as (first leaf,
+// leaves, node, depth) in any order: the treelets are disjoint.
+__global__ void __launch_bounds__(kBlk) k_treelet_roots(const uint2* __restrict__ range,
+                                                        const uint32_t* __restrict__ parent_int, uint32_t ni,
+                                                        uint4* __restrict__ list, uint32_t* __restrict__ count)
+{
+    const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+    if (i == 0u || i >= ni) return;
+    const uint2 ri = range[i];
+    const uint32_t m = ri.y - ri.x + 1u;
+    const uint32_t p = parent_int[i] & ~kLocalBit;
+    const uint2 rp = range[p];
+    if (m > kSahMaxPrims || rp.y - rp.x + 1u <= kSahMaxPrims) return;
+    uint32_t d = 1, q = p;
+    while (q != 0u && d <= kMaxDepth) {
+        q = parent_int[q] & ~kLocalBit;
+        d++;
+    }
+    if (d >= kSahDepthCap) return;
+    list[atomicAdd(count, 1u)] = make_uint4(ri.x, m, i, d);
+}
+
+__global__ void __launch_bounds__(kSahThreads) k_sah_treelets(const float4* __restrict__ sph,
+                                                              const uint4* __restrict__ list,
+                                                              const uint32_t* __restrict__ count,
+                                                              uint32_t* __restrict__ leaf_prim,
+                                                              uint32_t* __restrict__ keys,
+                                                              uint32_t* __restrict__ child,
+                                                              uint32_t* __restrict__ parent_int,
+                                                              uint32_t* __restrict__ parent_leaf)
+{
+    __shared__ SahShared sh;
+    const uint32_t cnt = *count;
+    for (uint32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+        const uint4 e = list[t];
+        sah_treelet(sph, e.x, e.y, e.z, e.w, leaf_prim, keys, child, parent_int, parent_leaf, sh);
     }
 }
 
@@ -1203,16 +1281,16 @@ vzStatus lbvh_build(vzScene s, hipStream_t st)
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in, n, fin, grid_b,
                        s->d_keys[0], s->d_vals[0]);
     s->builder = (n >= 2 && n <= kSahMaxPrims) ? VZ_BUILD_SAH : VZ_BUILD_LBVH;
+    // keys and leaf order end in buffers 0 either way
+    s->d_morton = s->d_keys[0];
+    s->d_leaf_prim = s->d_vals[0];
     if (s->builder == VZ_BUILD_SAH) {
-        // keys stay in sphere order in buffer 0; leaf order and its keys land in buffers 1
-        hipLaunchKernelGGL(k_sah_build, dim3(1), dim3(kSahThreads), 0, st, s->d_sph_in, n, s->d_keys[0],
-                           s->d_vals[1], s->d_keys[1], s->d_child, s->d_parent_int, s->d_parent_leaf);
-        s->d_morton = s->d_keys[1];
-        s->d_leaf_prim = s->d_vals[1];
+        hipLaunchKernelGGL(k_sah_build, dim3(1), dim3(kSahThreads), 0, st, s->d_sph_in, n, s->d_leaf_prim,
+                           s->d_morton, s->d_child, s->d_parent_int, s->d_parent_leaf);
     } else {
         uint32_t tiles = s->hist_blocks;
         uint32_t* dtot = s->d_hist + 256 * tiles;
-        for (int pass = 0; pass < 4; pass++) {
+        for (int pass = 0; pass < 4; pass++) {  // four passes: the result is back in buffers 0
             int src = pass & 1, dst = src ^ 1;
             hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src], n, pass * 8,
                                tiles, s->d_hist);
@@ -1220,15 +1298,26 @@ vzStatus lbvh_build(vzScene s, hipStream_t st)
             hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlk), 0, st, s->d_keys[src],
                                s->d_vals[src], n, pass * 8, tiles, s->d_hist, dtot, s->d_keys[dst], s->d_vals[dst]);
         }
-        s->d_morton = s->d_keys[0];  // four passes: result back in buffer 0
-        s->d_leaf_prim = s->d_vals[0];
+        if (n >= 2) {
+            // node leaf ranges go to the wide-children buffer, which the collapse rewrites later
+            uint2* range = reinterpret_cast<uint2*>(s->d_wkids);
+            hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, kBlk)), dim3(kBlk), 0, st, s->d_morton, n,
+                               s->d_child, s->d_parent_int, s->d_parent_leaf, s->d_visit, range);
+            if (n >= kTreeletMinPrims) {
+                // the treelet list uses the node-box buffer, which the refit fills afterwards
+                uint4* list = reinterpret_cast<uint4*>(s->d_boxes);
+                uint32_t* count = s->d_fcount + 8;
+                hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlk), 0, st, count, 1u);
+                hipLaunchKernelGGL(k_treelet_roots, dim3(blocks_for(n - 1, kBlk)), dim3(kBlk), 0, st, range,
+                                   s->d_parent_int, n - 1, list, count);
+                hipLaunchKernelGGL(k_sah_treelets, dim3(kTreeletBlocks), dim3(kSahThreads), 0, st, s->d_sph_in,
+                                   list, count, s->d_leaf_prim, s->d_morton, s->d_child, s->d_parent_int,
+                                   s->d_parent_leaf);
+            }
+        }
     }
     hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st, s->d_sph_in,
                        s->d_mat_in, s->d_leaf_prim, n, s->d_sph, s->d_leaf_info, s->d_mat);
-    if (n >= 2 && s->builder == VZ_BUILD_LBVH) {
-        hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, kBlk)), dim3(kBlk), 0, st, s->d_morton, n,
-                           s->d_child, s->d_parent_int, s->d_parent_leaf, s->d_visit);
-    }
     VZ_HIP_TRY(hipGetLastError());
     vzStatus rs = refit_core(s, st);
     VZ_HIP_TRY(hipEventRecord(e1.e, st));
