@@ -406,6 +406,7 @@ constexpr int kSahWaves = 8;
 constexpr int kSahThreads = 64 * kSahWaves;
 constexpr int kSahBinWords = 3 * kSahBins * 7;  // per wave: [axis][bin] lo xyz, hi xyz keys, count
 static_assert(kSahMaxPrims <= kRefitLeaves, "every SAH range lies in one refit workgroup");
+static_assert(kSahWaves * kSahBinWords >= (int)kSahMaxPrims, "the bins hold a treelet's keys at the end");
 static_assert(kSahDepthCap + 10 + 1 <= kMaxWideDepth, "SAH tree depth within the traversal stack");
 
 __device__ __forceinline__ uint32_t ord_key(float f)
@@ -444,19 +445,15 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
-// LDS of one treelet build: the range's primitives and keys by local
-// index, the local permutation, two level lists of ranges, and per wave the
-// bins and the sweep's suffix areas and counts.
+// LDS of one treelet build: the range's spheres by local index (gathered
+// once: every phase of every round reads them), the local permutation, two
+// level lists of ranges, and per wave the bins.
 struct SahShared {
-    uint32_t prim[kSahMaxPrims], key[kSahMaxPrims];
+    float4 sph[kSahMaxPrims];
     uint32_t perm[kSahMaxPrims], tmp[kSahMaxPrims];
     uint2 task[2][kSahMaxPrims / 2];  // (a | b << 10 local, node index)
     uint32_t ntask[2];
     uint32_t bin[kSahWaves][kSahBinWords];
-    float ra[kSahWaves][3][kSahBins];
-    uint32_t rn[kSahWaves][3][kSahBins];
-    float cost[kSahWaves][3];
-    int cbin[kSahWaves][3];
 };
 
 // Rebuilds the subtree over leaves [f, f + m) (2 <= m <= kSahMaxPrims), root
@@ -473,8 +470,7 @@ __device__ __forceinline__ void sah_treelet(const float4* __restrict__ sph, uint
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     for (uint32_t i = tid; i < m; i += kSahThreads) {
-        sh.prim[i] = leaf_prim[f + i];
-        sh.key[i] = keys[f + i];
+        sh.sph[i] = sph[leaf_prim[f + i]];
         sh.perm[i] = i;
     }
     if (tid == 0) {
@@ -496,7 +492,7 @@ __device__ __forceinline__ void sah_treelet(const float4* __restrict__ sph, uint
             float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
             if (has)
                 for (uint32_t i = a + lane; i <= b; i += 64u) {
-                    const float4 sp = sph[sh.prim[sh.perm[i]]];
+                    const float4 sp = sh.sph[sh.perm[i]];
                     for (int k = 0; k < 3; k++) {
                         lo[k] = fminf(lo[k], sah_cen(sp, k));
                         hi[k] = fmaxf(hi[k], sah_cen(sp, k));
@@ -516,7 +512,7 @@ __device__ __forceinline__ void sah_treelet(const float4* __restrict__ sph, uint
             __syncthreads();
             if (sah)
                 for (uint32_t i = a + lane; i <= b; i += 64u) {
-                    const float4 sp = sph[sh.prim[sh.perm[i]]];
+                    const float4 sp = sh.sph[sh.perm[i]];
                     const float rr = fabsf(sp.w);
                     const float lb[6] = {sp.x - rr, sp.y - rr, sp.z - rr, sp.x + rr, sp.y + rr, sp.z + rr};
                     for (int k = 0; k < 3; k++) {
@@ -530,72 +526,74 @@ __device__ __forceinline__ void sah_treelet(const float4* __restrict__ sph, uint
                     }
                 }
             __syncthreads();
-            // one lane per axis: suffix areas and counts, then the prefix sweep
-            if (lane < 3u) {
-                const int k = (int)lane;
-                int best = -1;
-                float bcost = 0.0f;
-                if (sah && ext[k] > 0.0f) {
-                    const uint32_t* bins = &sh.bin[w][k * kSahBins * 7];
-                    float R[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
-                    uint32_t nr = 0;
-                    for (int j = kSahBins - 1; j >= 1; j--) {
-                        const uint32_t c = bins[j * 7 + 6];
-                        if (c) {
-                            for (int q = 0; q < 3; q++) {
-                                R[q] = fminf(R[q], ord_val(bins[j * 7 + q]));
-                                R[3 + q] = fmaxf(R[3 + q], ord_val(bins[j * 7 + 3 + q]));
-                            }
-                        }
-                        nr += c;
-                        sh.ra[w][k][j] = box_area6(R);
-                        sh.rn[w][k][j] = nr;
+            // the cheapest split, per axis with one lane per bin: inclusive
+            // prefix and suffix unions (min/max, exact in any order) and
+            // counts over the 32 bins, the cost of the split after each bin,
+            // then the first minimum.  The unions start from the box
+            // (3.4e38, -3.4e38) as the sequential sweep of the oracle does.
+            int bk = -1, bj = -1;
+            if (sah) {
+                float bc = 0.0f;
+                const int j = (int)(lane & 31u);
+                for (int k = 0; k < 3; k++) {
+                    if (!(ext[k] > 0.0f)) continue;
+                    const uint32_t* bn = &sh.bin[w][(k * kSahBins + j) * 7];
+                    const uint32_t c = bn[6];
+                    float P[6], S[6];
+                    for (int q = 0; q < 3; q++) {
+                        P[q] = S[q] = c ? ord_val(bn[q]) : 3.4e38f;
+                        P[3 + q] = S[3 + q] = c ? ord_val(bn[3 + q]) : -3.4e38f;
                     }
-                    float L[6] = {3.4e38f, 3.4e38f, 3.4e38f, -3.4e38f, -3.4e38f, -3.4e38f};
-                    uint32_t nl = 0;
-                    for (int j = 0; j < kSahBins - 1; j++) {
-                        const uint32_t c = bins[j * 7 + 6];
-                        if (c) {
-                            for (int q = 0; q < 3; q++) {
-                                L[q] = fminf(L[q], ord_val(bins[j * 7 + q]));
-                                L[3 + q] = fmaxf(L[3 + q], ord_val(bins[j * 7 + 3 + q]));
-                            }
+                    uint32_t pc = c, sc = c;
+                    for (int d = 1; d < kSahBins; d <<= 1) {
+                        for (int q = 0; q < 6; q++) {
+                            const float up = __shfl_up(P[q], d, kSahBins), dn = __shfl_down(S[q], d, kSahBins);
+                            if (j >= d) P[q] = q < 3 ? fminf(P[q], up) : fmaxf(P[q], up);
+                            if (j + d < kSahBins) S[q] = q < 3 ? fminf(S[q], dn) : fmaxf(S[q], dn);
                         }
-                        nl += c;
-                        const uint32_t rn = sh.rn[w][k][j + 1];
-                        if (nl == 0u || rn == 0u) continue;
-                        const float cost = box_area6(L) * (float)nl + sh.ra[w][k][j + 1] * (float)rn;
-                        if (best < 0 || cost < bcost) {
-                            best = j;
-                            bcost = cost;
+                        const uint32_t upc = __shfl_up(pc, d, kSahBins), dnc = __shfl_down(sc, d, kSahBins);
+                        if (j >= d) pc += upc;
+                        if (j + d < kSahBins) sc += dnc;
+                    }
+                    float L[6], R[6];
+                    for (int q = 0; q < 6; q++) {
+                        const float r1 = __shfl_down(S[q], 1, kSahBins);  // bins j+1 .. 31
+                        L[q] = q < 3 ? fminf(3.4e38f, P[q]) : fmaxf(-3.4e38f, P[q]);
+                        R[q] = q < 3 ? fminf(3.4e38f, r1) : fmaxf(-3.4e38f, r1);
+                    }
+                    const uint32_t rn = __shfl_down(sc, 1, kSahBins);
+                    bool ok = j < kSahBins - 1 && pc > 0u && rn > 0u;
+                    float cost = ok ? box_area6(L) * (float)pc + box_area6(R) * (float)rn : 0.0f;
+                    int cj = j;
+                    for (int o = kSahBins / 2; o >= 1; o >>= 1) {
+                        const bool ook = __shfl_xor(ok ? 1 : 0, o, kSahBins) != 0;
+                        const float oc = __shfl_xor(cost, o, kSahBins);
+                        const int oj = __shfl_xor(cj, o, kSahBins);
+                        if ((ook && !ok) || (ook == ok && (oc < cost || (oc == cost && oj < cj)))) {
+                            ok = ook;
+                            cost = oc;
+                            cj = oj;
                         }
+                    }
+                    if (ok && (bk < 0 || cost < bc)) {
+                        bk = k;
+                        bj = cj;
+                        bc = cost;
                     }
                 }
-                sh.cost[w][k] = bcost;
-                sh.cbin[w][k] = best;
             }
-            __syncthreads();
-            int bk = -1, bj = -1;
-            float bc = 0.0f;
-            if (sah)
-                for (int k = 0; k < 3; k++)
-                    if (sh.cbin[w][k] >= 0 && (bk < 0 || sh.cost[w][k] < bc)) {
-                        bk = k;
-                        bj = sh.cbin[w][k];
-                        bc = sh.cost[w][k];
-                    }
             uint32_t g = a + (b - a + 1u) / 2u - 1u;
             if (bk >= 0) {
                 uint32_t cl = 0;
                 for (uint32_t i = a + lane; i <= b; i += 64u)
-                    cl += sah_bin(sah_cen(sph[sh.prim[sh.perm[i]]], bk), lo[bk], ext[bk]) <= bj ? 1u : 0u;
+                    cl += sah_bin(sah_cen(sh.sph[sh.perm[i]], bk), lo[bk], ext[bk]) <= bj ? 1u : 0u;
                 const uint32_t nl = wave_sum(cl);
                 uint32_t offl = a, offr = a + nl;
                 for (uint32_t i0 = a; i0 <= b; i0 += 64u) {
                     const uint32_t i = i0 + lane;
                     const bool valid = i <= b;
                     const uint32_t p = valid ? sh.perm[i] : 0u;
-                    const bool left = valid && sah_bin(sah_cen(sph[sh.prim[p]], bk), lo[bk], ext[bk]) <= bj;
+                    const bool left = valid && sah_bin(sah_cen(sh.sph[p], bk), lo[bk], ext[bk]) <= bj;
                     const uint64_t ml = __ballot(left), mr = __ballot(valid && !left);
                     if (left) sh.tmp[offl + mbcnt(ml)] = p;
                     else if (valid) sh.tmp[offr + mbcnt(mr)] = p;
@@ -624,10 +622,19 @@ __device__ __forceinline__ void sah_treelet(const float4* __restrict__ sph, uint
         cur ^= 1u;
     }
     __syncthreads();
+    // permute the range's primitive ids and keys in place, staged through
+    // LDS (the bins are free now) so every global read has landed before the
+    // barrier that precedes the writes
+    uint32_t* kbuf = &sh.bin[0][0];
     for (uint32_t i = tid; i < m; i += kSahThreads) {
         const uint32_t p = sh.perm[i];
-        leaf_prim[f + i] = sh.prim[p];
-        keys[f + i] = sh.key[p];
+        sh.tmp[i] = leaf_prim[f + p];
+        kbuf[i] = keys[f + p];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < m; i += kSahThreads) {
+        leaf_prim[f + i] = sh.tmp[i];
+        keys[f + i] = kbuf[i];
     }
     __syncthreads();  // the LDS is reused by the next treelet
 }
