@@ -91,8 +91,9 @@ int main(void)
     if (run_scene(bs, bm, nb, 0)) return 1;
     free(bs);
     free(bm);
-    /* box scenes of several sizes, including 1 and 2 spheres, duplicates */
-    const uint32_t sizes[] = {0, 1, 2, 3, 64, 2049, 5000};
+    /* box scenes of several sizes, including 1 and 2 spheres, duplicates,
+     * the largest SAH-built scene (1024) and an LBVH with SAH treelets */
+    const uint32_t sizes[] = {0, 1, 2, 3, 64, 2049, 5000, 1023, 131071};
     for (unsigned k = 0; k < sizeof(sizes) / sizeof(sizes[0]); k++) {
         uint32_t n = sizes[k] + 1;
         vzo_sphere* s = (vzo_sphere*)xalloc(sizeof(vzo_sphere) * n);

@@ -1,6 +1,7 @@
 """The CPU oracle under AddressSanitizer and UBSan: a host build of
 oracle/vz_oracle.c (gcc, plain C) runs every entry point on small scenes
-(the book scene, box scenes of 1 to 5001 spheres, duplicate centres, FP32
+(the book scene, box scenes of 1 to 5001 spheres and of 1024 and 131072
+(the SAH build and the SAH treelets), duplicate centres, FP32
 and FP64 accumulation, row subsets, the ray recorder, brute force and BVH
 traversal) from exact-size heap buffers; any out-of-bounds access or
 undefined behaviour aborts the run."""
