@@ -58,6 +58,7 @@ struct ReqDesc {             // device table entry, written by the host
     uint32_t spp, subframe, max_depth, flags;
     uint32_t npix, tw, th, tiles_x, npad;
     uint32_t K;              // pass ring slots
+    double rd_tsz, rd_tw, rd_tx, rd_K;  // 1 / (tw * th), 1 / tw, 1 / tiles_x, 1 / K (udiv_r)
     uint32_t seq;            // arrival number: free queue room goes to the lowest first
     unsigned long long total_items;  // spp * npad
     float* passbuf;          // K * 3 * npix, planar per slot
@@ -102,6 +103,8 @@ struct Plan {
     uint32_t regen_cnt[kMaxRequests];      // items granted to it (may span several passes)
     uint32_t run_prefix[kMaxRequests];     // first pixel run of it: min(cnt, npad) runs each
     unsigned long long regen_base[kMaxRequests];  // its first item (pass * npad + padded pixel)
+    uint32_t regen_s0[kMaxRequests];       // that item's pass
+    uint32_t regen_i0[kMaxRequests];       // and padded pixel
     uint32_t total_runs;
     uint32_t blk_first[kMaxBlocks];   // first granted index of each slice
     uint32_t blk_count[kMaxBlocks];   // granted items for each slice

@@ -283,33 +283,57 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
 // Exclusive scan over the 1024-thread control workgroup: an inclusive scan
 // inside each wave by shuffles, the 16 wave totals scanned by the first wave,
 // two barriers in all.  s needs 16 words.
-__device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s, uint32_t* total)
+// N independent scans share one set of barriers and shuffle steps: out[k]
+// is the exclusive prefix of v[k], tot[k] the workgroup total.  s needs
+// 33 * N words.
+template <int N>
+__device__ void block_excl_scan_n(const uint32_t (&v)[N], uint32_t (&out)[N], uint32_t (&tot)[N], uint32_t* s)
 {
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    uint32_t incl = v;
+    uint32_t incl[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) incl[k] = v[k];
 #pragma unroll
     for (uint32_t off = 1; off < 64u; off <<= 1) {
-        const uint32_t x = __shfl_up(incl, off);
-        if (lane >= off) incl += x;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const uint32_t x = __shfl_up(incl[k], off);
+            if (lane >= off) incl[k] += x;
+        }
     }
-    if (lane == 63u) s[wave] = incl;
+    if (lane == 63u)
+        for (int k = 0; k < N; k++) s[k * 33 + wave] = incl[k];
     __syncthreads();
     if (wave == 0) {
-        const uint32_t w = lane < 16u ? s[lane] : 0u;
-        uint32_t wi = w;
 #pragma unroll
-        for (uint32_t off = 1; off < 16u; off <<= 1) {
-            const uint32_t x = __shfl_up(wi, off);
-            if (lane >= off) wi += x;
+        for (int k = 0; k < N; k++) {
+            const uint32_t w = lane < 16u ? s[k * 33 + lane] : 0u;
+            uint32_t wi = w;
+#pragma unroll
+            for (uint32_t off = 1; off < 16u; off <<= 1) {
+                const uint32_t x = __shfl_up(wi, off);
+                if (lane >= off) wi += x;
+            }
+            if (lane < 16u) s[k * 33 + 16 + lane] = wi - w;  // exclusive prefix of the wave totals
+            if (lane == 15u) s[k * 33 + 32] = wi;
         }
-        if (lane < 16u) s[16 + lane] = wi - w;  // exclusive prefix of the wave totals
-        if (lane == 15u) s[32] = wi;
     }
     __syncthreads();
-    const uint32_t r = s[16 + wave] + incl - v;
-    *total = s[32];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        out[k] = s[k * 33 + 16 + wave] + incl[k] - v[k];
+        tot[k] = s[k * 33 + 32];
+    }
     __syncthreads();  // s is reused by the next scan
-    return r;
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s, uint32_t* total)
+{
+    const uint32_t in[1] = {v};
+    uint32_t out[1], tot[1];
+    block_excl_scan_n<1>(in, out, tot, s);
+    *total = tot[0];
+    return out[0];
 }
 
 
@@ -410,14 +434,14 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     __syncthreads();
     const uint32_t grant = before >= freeq ? 0u : (uint32_t)min((unsigned long long)quota, freeq - before);
     if (running && grant) st[r].next_item = next_item + grant;
-    uint32_t nA, totA, nR, totR, nact;
-    const uint32_t posA = block_excl_scan(acc_cnt ? 1u : 0u, s_scan, &nA);
-    const uint32_t pixA = block_excl_scan(acc_cnt ? npix : 0u, s_scan, &totA);
-    const uint32_t posR = block_excl_scan(grant ? 1u : 0u, s_scan, &nR);
-    const uint32_t preR = block_excl_scan(grant, s_scan, &totR);
-    uint32_t totRuns;
-    const uint32_t preRun = block_excl_scan(min(grant, npad), s_scan, &totRuns);  // pixels covered
-    (void)block_excl_scan(still ? 1u : 0u, s_scan, &nact);
+    // positions of the apply and regen entries, their pixel and ray
+    // prefixes, the pixels the grants cover, the requests still running
+    const uint32_t sv[6] = {acc_cnt ? 1u : 0u, acc_cnt ? npix : 0u, grant ? 1u : 0u, grant, min(grant, npad),
+                            still ? 1u : 0u};
+    uint32_t sp[6], stot[6];
+    block_excl_scan_n<6>(sv, sp, stot, s_scan);
+    const uint32_t posA = sp[0], pixA = sp[1], posR = sp[2], preR = sp[3], preRun = sp[4];
+    const uint32_t nA = stot[0], totA = stot[1], nR = stot[2], totR = stot[3], totRuns = stot[4], nact = stot[5];
     if (acc_cnt) {
         plan->acc_req[posA] = r;
         plan->acc_first[posA] = acc_first;
@@ -430,6 +454,9 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         plan->regen_cnt[posR] = grant;
         plan->run_prefix[posR] = preRun;
         plan->regen_base[posR] = next_item;
+        const unsigned long long s0 = next_item / npad;
+        plan->regen_s0[posR] = (uint32_t)s0;
+        plan->regen_i0[posR] = (uint32_t)(next_item - s0 * npad);
     }
     // 3. split the granted camera rays over the slices so that every slice
     // ends up near the same ray count: the next step launch lasts as long as
@@ -451,9 +478,10 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         my_g += g[q];
         my_short += g[q] < def[q] ? 1u : 0u;
     }
-    uint32_t tg, tshort;
-    (void)block_excl_scan(my_g, s_scan, &tg);
-    const uint32_t short_pre = block_excl_scan(my_short, s_scan, &tshort);
+    const uint32_t gv[2] = {my_g, my_short};
+    uint32_t gp[2], gtot[2];
+    block_excl_scan_n<2>(gv, gp, gtot, s_scan);
+    const uint32_t tg = gtot[0], short_pre = gp[1];
     // the rounding leftover (< number of short slices): one more ray each to
     // the first short slices
     const uint32_t left = totR - tg;
@@ -491,6 +519,19 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     }
 }
 
+// floor(n / d) for d >= 1 from rd = 1 / d rounded to double: n * rd is
+// within n / d * 2^-51 < 1 of n / d, so its truncation is off by at most one
+// either way, and one remainder test corrects it.  Exact for every 32-bit n;
+// a few instructions where an integer division takes dozens.
+__device__ __forceinline__ uint32_t udiv_r(uint32_t n, uint32_t d, double rd)
+{
+    uint32_t q = (uint32_t)((double)n * rd);
+    const long long r = (long long)n - (long long)q * d;
+    if (r < 0) q--;
+    else if (r >= (long long)d) q++;
+    return q;
+}
+
 // Largest e with prefix[e] <= j (prefix is ascending, prefix[0] == 0).
 __device__ __forceinline__ uint32_t find_entry(const uint32_t* prefix, uint32_t n, uint32_t j)
 {
@@ -517,9 +558,10 @@ __device__ __forceinline__ void apply_passes(const ReqDesc* __restrict__ req, co
         if (D.flags & VZ_ACCUM_FP64) {
             double s[3] = {0.0, 0.0, 0.0};
             if (first) for (int k = 0; k < 3; k++) s[k] = D.sum64[(size_t)k * D.npix + px];
-            for (uint32_t q = 0; q < c; q++) {
-                const float* pb = D.passbuf + (size_t)((first + q) % D.K) * 3u * D.npix;
+            for (uint32_t q = 0, slot = first - udiv_r(first, D.K, D.rd_K) * D.K; q < c; q++) {
+                const float* pb = D.passbuf + (size_t)slot * 3u * D.npix;
                 for (int k = 0; k < 3; k++) s[k] += (double)pb[(size_t)k * D.npix + px];
+                slot = slot + 1u == D.K ? 0u : slot + 1u;
             }
             if (!fin) {
                 for (int k = 0; k < 3; k++) D.sum64[(size_t)k * D.npix + px] = s[k];
@@ -538,9 +580,10 @@ __device__ __forceinline__ void apply_passes(const ReqDesc* __restrict__ req, co
         } else {
             float s[3] = {0.0f, 0.0f, 0.0f};
             if (first) for (int k = 0; k < 3; k++) s[k] = D.sum32[(size_t)k * D.npix + px];
-            for (uint32_t q = 0; q < c; q++) {
-                const float* pb = D.passbuf + (size_t)((first + q) % D.K) * 3u * D.npix;
+            for (uint32_t q = 0, slot = first - udiv_r(first, D.K, D.rd_K) * D.K; q < c; q++) {
+                const float* pb = D.passbuf + (size_t)slot * 3u * D.npix;
                 for (int k = 0; k < 3; k++) s[k] = s[k] + pb[(size_t)k * D.npix + px];
+                slot = slot + 1u == D.K ? 0u : slot + 1u;
             }
             if (!fin) {
                 for (int k = 0; k < 3; k++) D.sum32[(size_t)k * D.npix + px] = s[k];
@@ -588,18 +631,25 @@ __device__ __forceinline__ void regen_rays(const ReqDesc* __restrict__ req, cons
         const unsigned long long base = plan->regen_base[e];
         const unsigned long long end = base + plan->regen_cnt[e];
         const uint32_t jprefix = plan->regen_prefix[e];
-        unsigned long long item = base + (r - plan->run_prefix[e]);
-        uint32_t s = (uint32_t)(item / D.npad);
-        const uint32_t i = (uint32_t)(item - (unsigned long long)s * D.npad);
+        // item base + k = pass s, padded pixel i (k < npad: at most one wrap)
+        const uint32_t k = r - plan->run_prefix[e];
+        unsigned long long item = base + k;
+        uint32_t s = plan->regen_s0[e], i = plan->regen_i0[e] + k;
+        if (i >= D.npad) {
+            i -= D.npad;
+            s++;
+        }
         const uint32_t tsz = D.tw * D.th;
-        const uint32_t tile = i / tsz, w = i % tsz;
-        const uint32_t x = (tile % D.tiles_x) * D.tw + w % D.tw;
-        const uint32_t y = (tile / D.tiles_x) * D.th + w / D.tw;
+        const uint32_t tile = udiv_r(i, tsz, D.rd_tsz), w = i - tile * tsz;
+        const uint32_t ty = udiv_r(tile, D.tiles_x, D.rd_tx), wy = udiv_r(w, D.tw, D.rd_tw);
+        const uint32_t x = (tile - ty * D.tiles_x) * D.tw + (w - wy * D.tw);
+        const uint32_t y = ty * D.th + wy;
         const bool valid = x < D.cam.width && y < D.cam.height;
         const uint32_t pixel = valid ? y * D.cam.width + x : 0u;
         uint32_t seed = 0;
         if (valid) seed = s == 0u ? tea4(pixel, D.subframe) : D.camseed[pixel];
-        for (; item < end; item += D.npad, s++) {
+        uint32_t slot = s - udiv_r(s, D.K, D.rd_K) * D.K;
+        for (; item < end; item += D.npad, s++, slot = slot + 1u == D.K ? 0u : slot + 1u) {
             // granted index J lies in the last slice whose first index <= J
             const uint32_t J = jprefix + (uint32_t)(item - base);
             uint32_t lo = 0, hi = NB;
@@ -611,7 +661,7 @@ __device__ __forceinline__ void regen_rays(const ReqDesc* __restrict__ req, cons
             if (valid) {
                 v3 o, d;
                 camera_ray(D.cam, x, y, seed, o, d);
-                const uint32_t meta = (rq << 16) | ((s % D.K) << 8) | (D.max_depth - 1u);
+                const uint32_t meta = (rq << 16) | (slot << 8) | (D.max_depth - 1u);
                 n_o[at] = make_float4(o.x, o.y, o.z, __uint_as_float(pixel));
                 n_d[at] = make_float4(d.x, d.y, d.z, __uint_as_float(seed));
                 n_t[at] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(meta));
@@ -882,6 +932,10 @@ static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const v
     uint32_t tiles_y = (p.height + D.th - 1) / D.th;
     D.npad = D.tiles_x * D.tw * tiles_y * D.th;
     D.K = pass_slots(p);
+    D.rd_tsz = 1.0 / ((double)D.tw * D.th);
+    D.rd_tw = 1.0 / (double)D.tw;
+    D.rd_tx = 1.0 / (double)D.tiles_x;
+    D.rd_K = 1.0 / (double)D.K;
     D.seq = seq;
     D.total_items = (unsigned long long)D.spp * D.npad;
     D.accum = p.accum;
