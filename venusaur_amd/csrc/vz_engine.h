@@ -59,6 +59,8 @@ struct ReqDesc {             // device table entry, written by the host
     uint32_t npix, tw, th, tiles_x, npad;
     uint32_t K;              // pass ring slots
     double rd_tsz, rd_tw, rd_tx, rd_K;  // 1 / (tw * th), 1 / tw, 1 / tiles_x, 1 / K (udiv_r)
+    uint32_t ntiles, tstride;  // tiles per pass; the stride of the tile visiting order
+    double rd_ntiles;
     uint32_t seq;            // arrival number: free queue room goes to the lowest first
     unsigned long long total_items;  // spp * npad
     float* passbuf;          // K * 3 * npix, planar per slot

@@ -24,6 +24,7 @@ constexpr uint32_t kLdsNodes = 256;  // a scene of at most this many 4-wide node
 // node source, chosen by sweeping the 1080p book scene (LDS nodes) and
 // 1M-sphere box scenes (global nodes).
 constexpr uint32_t kRefillLds = 9;
+constexpr uint32_t kStrideMaxTiles = 24576;  // requests with fewer tiles visit them strided
 constexpr uint32_t kRefillGlobal = 41;
 constexpr int kRegenBlk = 256;
 constexpr uint32_t kRegenOver = 8;  // regen workgroups per queue slice: ~one pixel run per thread at 1080p
@@ -532,6 +533,18 @@ __device__ __forceinline__ uint32_t udiv_r(uint32_t n, uint32_t d, double rd)
     return q;
 }
 
+// n mod d for n < 2^53, d >= 1, rd = 1 / d rounded: n is exact in double and
+// n * rd is within n / d * 2^-52 < 1 of n / d (n / d < 2^52), so one
+// correction either way gives the exact remainder.
+__device__ __forceinline__ uint32_t umod_r(unsigned long long n, uint32_t d, double rd)
+{
+    const long long q = (long long)((double)n * rd);
+    long long r = (long long)n - q * (long long)d;
+    if (r < 0) r += d;
+    else if (r >= (long long)d) r -= d;
+    return (uint32_t)r;
+}
+
 // Largest e with prefix[e] <= j (prefix is ascending, prefix[0] == 0).
 __device__ __forceinline__ uint32_t find_entry(const uint32_t* prefix, uint32_t n, uint32_t j)
 {
@@ -640,7 +653,9 @@ __device__ __forceinline__ void regen_rays(const ReqDesc* __restrict__ req, cons
             s++;
         }
         const uint32_t tsz = D.tw * D.th;
-        const uint32_t tile = udiv_r(i, tsz, D.rd_tsz), w = i - tile * tsz;
+        const uint32_t t0 = udiv_r(i, tsz, D.rd_tsz), w = i - t0 * tsz;
+        // the t0-th tile visited
+        const uint32_t tile = umod_r((unsigned long long)t0 * D.tstride, D.ntiles, D.rd_ntiles);
         const uint32_t ty = udiv_r(tile, D.tiles_x, D.rd_tx), wy = udiv_r(w, D.tw, D.rd_tw);
         const uint32_t x = (tile - ty * D.tiles_x) * D.tw + (w - wy * D.tw);
         const uint32_t y = ty * D.th + wy;
@@ -931,6 +946,23 @@ static vzStatus setup_slot(Engine* e, uint32_t slot, uint32_t scene_idx, const v
     D.tiles_x = (p.width + D.tw - 1) / D.tw;
     uint32_t tiles_y = (p.height + D.th - 1) / D.th;
     D.npad = D.tiles_x * D.tw * tiles_y * D.th;
+    // Small requests visit their tiles in the order t * tstride mod ntiles,
+    // a stride near ntiles / golden ratio and coprime to ntiles: the run of
+    // consecutive items one slice receives then samples the whole image
+    // instead of one band of it, so slices of sky and slices of ground cost
+    // alike (64-scene batch of 256-1024 px requests, at most 16384 tiles:
+    // plain launches +14%, batched +7%).  Large requests keep tile order:
+    // their slices hold many bands each, and neighbouring tiles in one slice
+    // trace more coherently (1080p, 32400 tiles: 1-2.5% slower strided).
+    D.ntiles = D.tiles_x * tiles_y;
+    D.tstride = 1;
+    if (D.ntiles > 2u && D.ntiles < kStrideMaxTiles) {
+        uint32_t st0 = (uint32_t)((double)D.ntiles * 0.6180339887498949) | 1u;
+        auto gcd = [](uint32_t a, uint32_t b) { while (b) { uint32_t t = a % b; a = b; b = t; } return a; };
+        while (gcd(st0, D.ntiles) != 1u) st0++;
+        D.tstride = st0 % D.ntiles;
+    }
+    D.rd_ntiles = 1.0 / (double)D.ntiles;
     D.K = pass_slots(p);
     D.rd_tsz = 1.0 / ((double)D.tw * D.th);
     D.rd_tw = 1.0 / (double)D.tw;
