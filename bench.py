@@ -464,8 +464,8 @@ def run_served(ctx, n_requests, cap, load=3.0, rmse_samples=3, server_tuning=Non
     t_plain_end = time.perf_counter() - t_start
     want = [_sha(r.accum_host()) for r in reqs]
     # served: a client thread submits at the arrival times
-    # the server runs with the serving defaults (24M-ray queue, 8x8 tiles, queue
-    # split 8): its latency depends on the queue size, which the batch autotuner
+    # the server runs with the serving defaults (48M-ray queue, 8x8 tiles, queue
+    # split 4): its latency depends on the queue size, which the batch autotuner
     # does not weigh
     srv = api.Server(ctx, hbm_budget_bytes=64 << 30, max_active=512,
                      tuning=server_tuning if server_tuning is not None else api.Tuning())

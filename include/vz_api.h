@@ -194,7 +194,7 @@ typedef struct vzTuning {
                                  still trace: 1 = only when all are done, 65 = after every
                                  node step, 0 = the default for the scene's node placement */
     uint32_t slice_split;     /* queue slices per resident step workgroup: 1, 2, 4 or 8 (0 = 4 for
-                                 batches, 8 for servers).  Slices of uneven cost (many scenes
+                                 batches and servers).  Slices of uneven cost (many scenes
                                  in one run) balance better when split; vzLaunch always uses 1 */
 } vzTuning;
 
@@ -204,7 +204,7 @@ vzStatus vzBatchLaunch(vzContext ctx, const vzBatchItem* items, uint32_t n,
                        const vzTuning* tuning, vzStream stream, vzStats* stats);
 /* Renders the batch twice per candidate tuning, timing the wavefront only,
  * and returns the candidate with the fastest run.  Stage 1: queue capacity
- * 4M/8M/12M/24M rays x camera tiles 32x2/16x4/8x8 (default refill, queue
+ * 4M/8M/12M/24M/32M/48M rays x camera tiles 32x2/16x4/8x8 (default refill, queue
  * split 4); stage 2, at the best of those: refill_lanes 25/33/49/57, then
  * slice_split 1/2/8.  The choice becomes this context's default and is cached
  * for the process per kind of device (name and CU count): contexts on the
@@ -221,8 +221,8 @@ typedef struct vzServeConfig {
                                     camera seeds).  Scenes are the caller's and already resident;
                                     they are not counted (vzAccelInfo.device_bytes gives theirs) */
     uint32_t max_active;         /* max requests in the device table (<= 1024) */
-    vzTuning tuning;             /* 0 fields take the serving defaults: 24M-ray queue, 8x8 tiles,
-                                    queue split 8 */
+    vzTuning tuning;             /* 0 fields take the serving defaults: 48M-ray queue, 8x8 tiles,
+                                    queue split 4 */
 } vzServeConfig;
 
 typedef struct vzRequestResult {

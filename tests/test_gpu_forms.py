@@ -218,7 +218,7 @@ def test_autotune_picks_a_candidate_and_keeps_results(ctx, scenes, oracle_result
     say("case: autotune")
     items, tgts = _items(scenes)
     t = api.autotune(ctx, items)
-    assert t.queue_capacity in (1 << 22, 1 << 23, 3 << 22, 6 << 22)
+    assert t.queue_capacity in (1 << 22, 1 << 23, 3 << 22, 6 << 22, 8 << 22, 12 << 22)
     assert (t.tile_w, t.tile_h) in ((32, 2), (16, 4), (8, 8))
     assert t.refill_lanes in (0, 25, 33, 49, 57)
     assert t.slice_split in (1, 2, 4, 8)

@@ -39,11 +39,12 @@ constexpr uint32_t kMaxSplit = 8;       // queue slices per resident step workgr
 // for 2 and 2712 for 1; plain launches of one scene keep 1 (1M box scene 2434
 // against 2259 at 4).
 constexpr uint32_t kBatchSplit = 4;
-// Servers split 8 ways: on the 2000-request stream's mix (5% 512x512 2048-spp
+// Servers split 4 ways: on the 2000-request stream's mix (5% 512x512 2048-spp
 // FP64 requests among 256-1024 px ones), all 600 first requests in flight at
-// once, a 24M-ray queue served 5037 Mray-segments/s at split 8 against 4699
-// for the 12M queue at split 4, 4537 for 12M at 8 and 4088 for 12M at 2.
-constexpr uint32_t kServeSplit = 8;
+// once, with small requests visiting their tiles strided, a 48M-ray queue
+// served 5648 Mray-segments/s at split 4 against 5518 at split 8; 32M: 5576
+// at 4, 5424 at 8, 5367 at 2; 24M: 5303 at 4, 5087 at 8.
+constexpr uint32_t kServeSplit = 4;
 constexpr uint32_t kDeadRay = 0xFFFFFFFFu;  // meta of a padding item: skipped
 // Scene table entries, allocated once when the engine is created, so that the
 // table's address never changes: captured hipGraphs bake it into their step

@@ -219,14 +219,12 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     if (s->cfg.max_active == 0 || s->cfg.max_active > kMaxRequests) s->cfg.max_active = kMaxRequests;
     if (s->cfg.hbm_budget_bytes == 0) s->cfg.hbm_budget_bytes = ctx->hbm_bytes / 2;
     EngineConfig ec;
-    // Serving defaults: a 24M-ray queue, 8x8 camera tiles and the queue split
-    // 8 ways per resident step workgroup (kServeSplit).  On the 120-request
-    // stream at 3x the plain capacity, 12M had served 3810 Mray-segments/s
-    // against 3631 for 8M and 3798 for 24M (32x2 tiles at 8M: 3595); on the
-    // 2000-request stream, whose 512x512 2048-spp requests hold most of the
-    // work, the larger queue wins (24M at split 8: 5037 against 4699 for 12M
-    // at split 4).
-    ec.queue_capacity = 6u << 22;
+    // Serving defaults: a 48M-ray queue (4.8 GB for its two parities), 8x8
+    // camera tiles and the queue split 4 ways per resident step workgroup
+    // (kServeSplit).  On the first 600 requests of the 2000-request stream,
+    // all in flight at once: 5648 Mray-segments/s against 5576 for 32M, 5303
+    // for 24M and 5690 for 64M, all at split 4 (16x4 tiles at 24M: 5020).
+    ec.queue_capacity = 12u << 22;
     ec.tile_w = 8;
     ec.tile_h = 8;
     if (s->cfg.tuning.queue_capacity) ec.queue_capacity = s->cfg.tuning.queue_capacity;
@@ -347,7 +345,7 @@ vzStatus vzAutotune(vzContext ctx, const vzBatchItem* items, uint32_t n, vzStrea
         return VZ_SUCCESS;
     };
     // stage 1: queue capacity x camera tile at the default refill and split
-    const uint32_t caps[] = {1u << 22, 1u << 23, 3u << 22, 6u << 22};
+    const uint32_t caps[] = {1u << 22, 1u << 23, 3u << 22, 6u << 22, 8u << 22, 12u << 22};
     const uint32_t tiles[][2] = {{32, 2}, {16, 4}, {8, 8}};
     for (uint32_t cap : caps)
         for (auto& t : tiles) {
