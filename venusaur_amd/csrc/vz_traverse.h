@@ -131,10 +131,14 @@ struct Traversal {
         float lo_t = tmin, hi_t = t;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const float t0 = ((c[k] - r) - org[k]) * inv[k];
-            const float t1 = ((c[k] + r) - org[k]) * inv[k];
-            lo_t = fmaxf(lo_t, fminf(t0, t1));
-            hi_t = fminf(hi_t, fmaxf(t0, t1));
+            // the near plane is c - r for a positive inv, c + r for a negative
+            // one: with sr = copysign(r, inv), c - sr and c + sr are exactly
+            // those planes (x - (-y) rounds as x + y), and rounding is
+            // monotone, so these are min(t0, t1) and max(t0, t1) of the two
+            // plane distances bit for bit
+            const float sr = copysignf(r, inv[k]);
+            lo_t = fmaxf(lo_t, ((c[k] - sr) - org[k]) * inv[k]);
+            hi_t = fminf(hi_t, ((c[k] + sr) - org[k]) * inv[k]);
         }
         return lo_t <= hi_t * kRobust;
     }
