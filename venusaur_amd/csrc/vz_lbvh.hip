@@ -456,6 +456,38 @@ struct SahShared {
     uint32_t bin[kSahWaves][kSahBinWords];
 };
 
+// The SAH split of a range of two spheres [x, x + 1] (node `self`: the
+// range's right end for a left child, its left end for a right child) at
+// depth d, in closed form: the two leaf boxes are the two sides whatever the axis,
+// so every axis of positive centre extent costs area(A) + area(B) alike and
+// the first one wins; the sphere with the smaller centre on it goes left.
+// With no such axis, or from the depth cap on, the order stays.  This is
+// what the binned sweep gives for two spheres (they fall in bins 0 and 31).
+__device__ __forceinline__ void sah_pair(uint32_t x, uint32_t f, uint32_t self, uint32_t d,
+                                         uint32_t* __restrict__ child, uint32_t* __restrict__ parent_leaf,
+                                         SahShared& sh)
+{
+    const uint32_t X = f + x;  // split after leaf X: both children leaves
+    const float4 s0 = sh.sph[sh.perm[x]], s1 = sh.sph[sh.perm[x + 1u]];
+    if (d < kSahDepthCap) {
+        for (int k = 0; k < 3; k++) {
+            const float c0 = sah_cen(s0, k), c1 = sah_cen(s1, k);
+            if (c0 != c1) {
+                if (c0 > c1) {
+                    const uint32_t t = sh.perm[x];
+                    sh.perm[x] = sh.perm[x + 1u];
+                    sh.perm[x + 1u] = t;
+                }
+                break;
+            }
+        }
+    }
+    const uint32_t up = self | (((X / kRefitLeaves) == ((X + 1u) / kRefitLeaves)) ? kLocalBit : 0u);
+    parent_leaf[X] = up;
+    parent_leaf[X + 1u] = up;
+    child[self] = X | (3u << 30);
+}
+
 // Rebuilds the subtree over leaves [f, f + m) (2 <= m <= kSahMaxPrims), root
 // node r at depth d0, in place: leaf_prim and keys are permuted within the
 // range, and the child words and parent links of the subtree's nodes are
@@ -612,9 +644,18 @@ __device__ __forceinline__ void sah_treelet(const float4* __restrict__ sph, uint
                 if (g == a) { fl |= 1u; parent_leaf[G] = up; } else { parent_int[G] = up; }
                 if (g + 1u == b) { fl |= 2u; parent_leaf[G + 1u] = up; } else { parent_int[G + 1u] = up; }
                 child[self] = G | (fl << 30);
-                if (g > a) sh.task[cur ^ 1u][atomicAdd(&sh.ntask[cur ^ 1u], 1u)] = make_uint2(a | (g << 10), G);
-                if (g + 1u < b)
-                    sh.task[cur ^ 1u][atomicAdd(&sh.ntask[cur ^ 1u], 1u)] = make_uint2((g + 1u) | (b << 10), G + 1u);
+                // children of two spheres are finished here; larger ones are
+                // the next level's ranges
+                const uint32_t ca[2] = {a, g + 1u}, cb[2] = {g, b};
+                for (int c = 0; c < 2; c++) {
+                    if (cb[c] == ca[c]) continue;
+                    const uint32_t node = f + (c == 0 ? g : g + 1u);  // Karras layout index
+                    if (cb[c] == ca[c] + 1u)
+                        sah_pair(ca[c], f, node, d0 + level + 1u, child, parent_leaf, sh);
+                    else
+                        sh.task[cur ^ 1u][atomicAdd(&sh.ntask[cur ^ 1u], 1u)] =
+                            make_uint2(ca[c] | (cb[c] << 10), node);
+                }
             }
         }
         __syncthreads();
