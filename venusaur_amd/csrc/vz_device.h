@@ -169,17 +169,22 @@ VZ_DEV v3 sky(v3 d)
 VZ_DEV bool scatter(uint32_t type, float4 mat, v3 d, v3 n, bool front, uint32_t& seed, v3& nd,
                     v3& att)
 {
+    // Lambertian and metal both draw a point in the unit sphere (metal's
+    // reflection draws nothing) and metal and dielectric both normalise d:
+    // each is done once for the lanes that need it, so a wave holding several
+    // materials runs one rejection loop and one normalisation instead of two
+    v3 p = mk(0.0f, 0.0f, 0.0f);
+    if (type != 2u) p = in_unit_sphere(seed);
     if (type == 0u) {
-        v3 us = normalize(in_unit_sphere(seed));
-        v3 sd = add(n, us);
+        v3 sd = add(n, normalize(p));
         if (near_zero(sd)) sd = n;
         nd = sd;
         att = mk(mat.x, mat.y, mat.z);
         return true;
-    } else if (type == 1u) {
-        v3 refl = reflect(normalize(d), n);
-        v3 rs = in_unit_sphere(seed);
-        v3 sd = add(refl, scl(rs, mat.w));
+    }
+    const v3 u = normalize(d);
+    if (type == 1u) {
+        v3 sd = add(reflect(u, n), scl(p, mat.w));
         if (dot(sd, n) > 0.0f) {
             nd = sd;
             att = mk(mat.x, mat.y, mat.z);
@@ -189,7 +194,6 @@ VZ_DEV bool scatter(uint32_t type, float4 mat, v3 d, v3 n, bool front, uint32_t&
     } else {
         float ir = mat.w;
         float ratio = front ? (1.0f / ir) : ir;
-        v3 u = normalize(d);
         float ct = fminf(dot(neg(u), n), 1.0f);
         float st = sqrtf(1.0f - ct * ct);
         bool cannot = ratio * st > 1.0f;
