@@ -25,6 +25,7 @@ constexpr uint32_t kLdsNodes = 256;  // a scene of at most this many 4-wide node
 // 1M-sphere box scenes (global nodes).
 constexpr uint32_t kRefillLds = 9;
 constexpr uint32_t kStrideMaxTiles = 24576;  // requests with fewer tiles visit them strided
+constexpr uint32_t kFlushDen = 2;  // queued sphere tests run once 1/kFlushDen of the tracing lanes hold some
 constexpr uint32_t kRefillGlobal = 41;
 constexpr int kRegenBlk = 256;
 constexpr uint32_t kRegenOver = 8;  // regen workgroups per queue slice: ~one pixel run per thread at 1080p
@@ -72,13 +73,13 @@ __device__ __forceinline__ void terminate(const ReqDesc& D, uint32_t slot, uint3
 }
 
 // One node visit: the 4-wide tree staged in LDS (LDSN) or read from memory.
-template <bool LDSN, int S, bool COUNT>
-__device__ __forceinline__ bool step_node(Traversal<S, COUNT>& tr, const SceneView& sc, const uint4* s_bvh,
+template <bool LDSN, int S, bool COUNT, bool DEFER>
+__device__ __forceinline__ bool step_node(Traversal<S, COUNT, DEFER>& tr, const SceneView& sc, const uint4* s_bvh,
                                           uint32_t* lds_node, float* lds_t, Overflow& ov, uint32_t& nv,
-                                          uint32_t& ns)
+                                          uint32_t& ns, uint32_t* lds_pend)
 {
-    if (LDSN) return tr.step(sc, LdsNodes{s_bvh}, lds_node, lds_t, kStepBlk, ov, nv, ns);
-    return tr.step(sc, GlobalNodes{sc.nodes}, lds_node, lds_t, kStepBlk, ov, nv, ns);
+    if (LDSN) return tr.step(sc, LdsNodes{s_bvh}, lds_node, lds_t, kStepBlk, ov, nv, ns, lds_pend);
+    return tr.step(sc, GlobalNodes{sc.nodes}, lds_node, lds_t, kStepBlk, ov, nv, ns, lds_pend);
 }
 
 // ---- step: trace + miss/shade stages + compaction, one slice per workgroup ----
@@ -108,6 +109,11 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
     __shared__ uint32_t s_head, s_out;
     __shared__ unsigned long long s_red[4][kStepBlk / 64];
     __shared__ uint4 s_bvh[LDSN ? 4 * kLdsNodes : 1];
+    // nodes from memory, timed runs: sphere tests queued per lane and run by
+    // the whole wave together (counting runs test at once, so their counts
+    // follow each ray alone)
+    constexpr bool DEFER = !COUNT && !LDSN;
+    __shared__ uint32_t s_pend[DEFER ? kPend * kStepBlk : 1];
     const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
     if (LDSN) {  // one small scene: stage its node array in LDS
         const uint32_t nn = 4u * scenes[0].nw;
@@ -130,7 +136,7 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
     uint32_t nv = 0, ns = 0, nh = 0, seg = 0;
     SceneView sc;
     if (SINGLE) sc = scenes[0];
-    Traversal<kStackS, COUNT> tr;
+    Traversal<kStackS, COUNT, DEFER> tr;
     Overflow ov;
     bool active = false, tdone = false;
     size_t qi = 0;
@@ -167,11 +173,23 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
         // of many lanes together, was measured slower at every threshold.)
         uint64_t busy = __ballot(active && !tdone);
         while (busy) {
-            if (active && !tdone) tdone = step_node<LDSN>(tr, sc, s_bvh, s_node + tid, s_t + tid, ov, nv, ns);
+            if (active && !tdone)
+                tdone = step_node<LDSN>(tr, sc, s_bvh, s_node + tid, s_t + tid, ov, nv, ns, s_pend + tid);
             busy = __ballot(active && !tdone);
+            if (DEFER) {
+                // flush once enough lanes hold queued tests, relative to the
+                // lanes still tracing, or before a queue could overflow (a
+                // visit queues at most 4)
+                const uint64_t pm = __ballot(tr.npend != 0u);
+                if (pm && (__ballot(tr.npend + 4u > kPend) != 0ull ||
+                           kFlushDen * (uint32_t)__popcll(pm) >= (uint32_t)__popcll(busy)))
+                    tr.flush(sc, s_pend + tid, kStepBlk, ns);
+            }
             // refill <= 1: the whole wave finishes before going back for rays
             if (!exhausted && (uint32_t)__popcll(busy) < refill) break;
         }
+        // every queued test has run before a lane's hit is shaded
+        if (DEFER) tr.flush(sc, s_pend + tid, kStepBlk, ns);
         // miss and shade stages for the lanes whose ray finished
         const bool fin = active && tdone;
         if (__ballot(fin) == 0ull) continue;

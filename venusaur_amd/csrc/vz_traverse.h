@@ -69,13 +69,24 @@ struct Overflow {
 // as its traversal ends instead of waiting for the slowest lane.
 // S: LDS slots per lane (power of two); lds_node/lds_t point at this lane's
 // column, consecutive slots `stride` words apart.
-template <int S, bool COUNT>
+// DEFER: leaf slots that a visit hits are queued (the lane's column of
+// lds_pend, at most kPend entries) instead of tested at once, and flush()
+// tests them; the caller flushes when many lanes of the wave hold some, so a
+// wave's sphere tests run with more lanes at a time.  Culling meanwhile uses
+// the best hit found so far, which is no smaller than it would otherwise be:
+// no node that holds the closest hit is culled, and the closest hit (the
+// least (t, prim) over the spheres tested) does not depend on the order of
+// the tests.  Node-visit and sphere-test counts do depend on when the wave
+// flushes, so the counting kernels test at once.
+constexpr uint32_t kPend = 8;
+template <int S, bool COUNT, bool DEFER = false>
 struct Traversal {
     v3 o, d;
     float inv[3];
     float tmin, best;
     uint32_t negm;  // bit k: the ray runs towards -k, so a slot's near plane on axis k is its hi plane
     uint32_t bleaf, bprim, node, sp, bottom;
+    uint32_t npend = 0;  // DEFER: queued leaf tests (0 also for a lane that never gets a ray)
 
     // Returns true when the ray is already finished (single-sphere scene).
     VZ_DEV bool init(const SceneView& sc, v3 o_, v3 d_, float tmin_, float tmax_, uint32_t& ns)
@@ -86,6 +97,7 @@ struct Traversal {
         best = tmax_;
         bleaf = 0xFFFFFFFFu;
         bprim = 0xFFFFFFFFu;
+        npend = 0;
         const float dd[3] = {d.x, d.y, d.z};
         negm = 0u;
 #pragma unroll
@@ -190,19 +202,30 @@ struct Traversal {
         return true;
     }
 
+    // DEFER: tests the queued leaves (the lane's column of lds_pend).
+    VZ_DEV void flush(const SceneView& sc, const uint32_t* lds_pend, uint32_t stride, uint32_t& ns)
+    {
+        while (npend) {
+            npend--;
+            const uint32_t leaf = lds_pend[npend * stride];
+            test_leaf(sc, leaf, ldg(sc.spheres + leaf), ns);
+        }
+    }
+
     // One wide node visit; returns true when the traversal has finished.
     template <class NodeSrc>
     VZ_DEV bool step(const SceneView& sc, const NodeSrc& nodes, uint32_t* lds_node, float* lds_t,
-                     uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns)
+                     uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns, uint32_t* lds_pend = nullptr)
     {
         uint4 A, B, C, D;
         nodes.fetch(node, A, B, C, D);
-        return visit(sc, A, B, C, D, lds_node, lds_t, stride, ov, nv, ns);
+        return visit(sc, A, B, C, D, lds_node, lds_t, stride, ov, nv, ns, lds_pend);
     }
 
     // The visit of the current node, whose 64 bytes are A..D.
     VZ_DEV bool visit(const SceneView& sc, const uint4& A, const uint4& B, const uint4& C, const uint4& D,
-                      uint32_t* lds_node, float* lds_t, uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns)
+                      uint32_t* lds_node, float* lds_t, uint32_t stride, Overflow& ov, uint32_t& nv, uint32_t& ns,
+                      uint32_t* lds_pend)
     {
         if (COUNT) nv++;
         const float no[3] = {__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z)};
@@ -260,7 +283,12 @@ struct Traversal {
             const uint32_t c = (uint32_t)__builtin_ctz(lm);
             const uint32_t lo = (c & 1u) ? ch[1] : ch[0], hi = (c & 1u) ? ch[3] : ch[2];
             const uint32_t leaf = (c & 2u) ? hi : lo;  // selects, not branches
-            test_leaf(sc, leaf, ldg(sc.spheres + leaf), ns);
+            if (DEFER) {
+                lds_pend[npend * stride] = leaf;
+                npend++;
+            } else {
+                test_leaf(sc, leaf, ldg(sc.spheres + leaf), ns);
+            }
             lm &= lm - 1u;
         }
         // internal slots that were hit and can still hold a closer hit
