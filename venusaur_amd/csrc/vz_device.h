@@ -87,12 +87,33 @@ VZ_DEV uint32_t tea4(uint32_t v0, uint32_t v1)
     }
     return v0;
 }
+// The LCG s <- 1664525 s + 1013904223 (mod 2^32), tracked mod 2^24: a draw
+// reads only the state's low 24 bits, and those depend only on the low 24
+// bits of the previous state (1664525 < 2^24), so the full-rate 24-bit
+// multiply-add v_mad_u32_u24 gives every draw of the 32-bit recurrence bit
+// for bit, where a 32-bit multiply (v_mul_lo_u32) issues at a quarter of the
+// rate.  The high 8 bits of s are not the recurrence's and nothing reads
+// them.  Written as one VALU instruction because the compiler, seeing that
+// only the low 24 bits are used, turns a 24-bit multiply back into a 32-bit
+// one; VOP3 takes no literal here, so the multiplier comes from an SGPR.
+VZ_DEV uint32_t lcg_next(uint32_t s)
+{
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(s), "s"(1664525u), "v"(1013904223u));
+    return r;
+}
 VZ_DEV float rnd(uint32_t& s)
 {
-    s = 1664525u * s + 1013904223u;
+    s = lcg_next(s);
     return (float)(s & 0x00FFFFFFu) * (1.0f / 16777216.0f);  // exact: 24-bit int * 2^-24
 }
-VZ_DEV float rnd_m11(uint32_t& s) { return -1.0f + 2.0f * rnd(s); }
+// -1 + 2 * rnd(s): q * 2^-24 and its double are exact, so the one rounding
+// of the sum is the fma's
+VZ_DEV float rnd_m11(uint32_t& s)
+{
+    s = lcg_next(s);
+    return __builtin_fmaf((float)(s & 0x00FFFFFFu), 0x1p-23f, -1.0f);
+}
 
 VZ_DEV v3 in_unit_sphere(uint32_t& s)
 {
