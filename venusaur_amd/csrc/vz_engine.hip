@@ -575,65 +575,89 @@ __device__ __forceinline__ uint32_t find_entry(const uint32_t* prefix, uint32_t 
 }
 
 // ---- apply: ordered per-pixel accumulation and the final blend/encode --------
+// Pixel px of apply entry e: adds the entry's finished passes in sample order
+// to the running sums, and on the request's last pass blends the average
+// into the accumulator and encodes the image.
+__device__ __forceinline__ void apply_pixel(const ReqDesc& D, const Plan* __restrict__ plan, uint32_t e, uint32_t px)
+{
+    const uint32_t first = plan->acc_first[e], c = plan->acc_cnt[e];
+    const bool fin = first + c == D.spp;
+    float oc[3];
+    if (D.flags & VZ_ACCUM_FP64) {
+        double s[3] = {0.0, 0.0, 0.0};
+        if (first) for (int k = 0; k < 3; k++) s[k] = D.sum64[(size_t)k * D.npix + px];
+        for (uint32_t q = 0, slot = first - udiv_r(first, D.K, D.rd_K) * D.K; q < c; q++) {
+            const float* pb = D.passbuf + (size_t)slot * 3u * D.npix;
+            for (int k = 0; k < 3; k++) s[k] += (double)pb[(size_t)k * D.npix + px];
+            slot = slot + 1u == D.K ? 0u : slot + 1u;
+        }
+        if (!fin) {
+            for (int k = 0; k < 3; k++) D.sum64[(size_t)k * D.npix + px] = s[k];
+            return;
+        }
+        double* acc = (double*)D.accum + 4 * (size_t)px;
+        double avg[3];
+        for (int k = 0; k < 3; k++) avg[k] = s[k] / (double)D.spp;
+        if (D.subframe > 1u) {
+            double a = 1.0 / (double)D.subframe;
+            for (int k = 0; k < 3; k++) avg[k] = acc[k] + a * (avg[k] - acc[k]);
+        }
+        reinterpret_cast<double2*>(acc)[0] = make_double2(avg[0], avg[1]);
+        reinterpret_cast<double2*>(acc)[1] = make_double2(avg[2], 1.0);
+        for (int k = 0; k < 3; k++) oc[k] = (float)avg[k];
+    } else {
+        float s[3] = {0.0f, 0.0f, 0.0f};
+        if (first) for (int k = 0; k < 3; k++) s[k] = D.sum32[(size_t)k * D.npix + px];
+        for (uint32_t q = 0, slot = first - udiv_r(first, D.K, D.rd_K) * D.K; q < c; q++) {
+            const float* pb = D.passbuf + (size_t)slot * 3u * D.npix;
+            for (int k = 0; k < 3; k++) s[k] = s[k] + pb[(size_t)k * D.npix + px];
+            slot = slot + 1u == D.K ? 0u : slot + 1u;
+        }
+        if (!fin) {
+            for (int k = 0; k < 3; k++) D.sum32[(size_t)k * D.npix + px] = s[k];
+            return;
+        }
+        float4* acc = (float4*)D.accum + px;
+        float inv = 1.0f / (float)D.spp;
+        float avg[3] = {s[0] * inv, s[1] * inv, s[2] * inv};
+        if (D.subframe > 1u) {
+            float4 prev = *acc;
+            float a = 1.0f / (float)D.subframe;
+            avg[0] = prev.x + a * (avg[0] - prev.x);
+            avg[1] = prev.y + a * (avg[1] - prev.y);
+            avg[2] = prev.z + a * (avg[2] - prev.z);
+        }
+        *acc = make_float4(avg[0], avg[1], avg[2], 1.0f);
+        for (int k = 0; k < 3; k++) oc[k] = avg[k];
+    }
+    if (D.image) D.image[px] = make_color(oc[0], oc[1], oc[2]);
+}
+
+// The first lane's value of v among the lanes where v == that value, as a
+// wave-uniform value: the caller runs a waterfall loop in which every active
+// lane has v == readfirstlane(v), and the compiler, knowing that equality,
+// would otherwise use the per-lane v; the empty asm hides the copy it reads.
+__device__ __forceinline__ uint32_t uniform_copy(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 __device__ __forceinline__ void apply_passes(const ReqDesc* __restrict__ req, const Plan* __restrict__ plan,
                                              uint32_t bid, uint32_t nblk)
 {
     const uint32_t total = plan->total_acc_pix, n = plan->n_acc;
     for (uint32_t j = bid * kBlk + threadIdx.x; j < total; j += nblk * kBlk) {
-        uint32_t e = find_entry(plan->acc_prefix, n, j);
-        const ReqDesc& D = req[plan->acc_req[e]];
-        uint32_t px = j - plan->acc_prefix[e];
-        uint32_t first = plan->acc_first[e], c = plan->acc_cnt[e];
-        bool fin = first + c == D.spp;
-        float oc[3];
-        if (D.flags & VZ_ACCUM_FP64) {
-            double s[3] = {0.0, 0.0, 0.0};
-            if (first) for (int k = 0; k < 3; k++) s[k] = D.sum64[(size_t)k * D.npix + px];
-            for (uint32_t q = 0, slot = first - udiv_r(first, D.K, D.rd_K) * D.K; q < c; q++) {
-                const float* pb = D.passbuf + (size_t)slot * 3u * D.npix;
-                for (int k = 0; k < 3; k++) s[k] += (double)pb[(size_t)k * D.npix + px];
-                slot = slot + 1u == D.K ? 0u : slot + 1u;
-            }
-            if (!fin) {
-                for (int k = 0; k < 3; k++) D.sum64[(size_t)k * D.npix + px] = s[k];
-                continue;
-            }
-            double* acc = (double*)D.accum + 4 * (size_t)px;
-            double avg[3];
-            for (int k = 0; k < 3; k++) avg[k] = s[k] / (double)D.spp;
-            if (D.subframe > 1u) {
-                double a = 1.0 / (double)D.subframe;
-                for (int k = 0; k < 3; k++) avg[k] = acc[k] + a * (avg[k] - acc[k]);
-            }
-            reinterpret_cast<double2*>(acc)[0] = make_double2(avg[0], avg[1]);
-            reinterpret_cast<double2*>(acc)[1] = make_double2(avg[2], 1.0);
-            for (int k = 0; k < 3; k++) oc[k] = (float)avg[k];
-        } else {
-            float s[3] = {0.0f, 0.0f, 0.0f};
-            if (first) for (int k = 0; k < 3; k++) s[k] = D.sum32[(size_t)k * D.npix + px];
-            for (uint32_t q = 0, slot = first - udiv_r(first, D.K, D.rd_K) * D.K; q < c; q++) {
-                const float* pb = D.passbuf + (size_t)slot * 3u * D.npix;
-                for (int k = 0; k < 3; k++) s[k] = s[k] + pb[(size_t)k * D.npix + px];
-                slot = slot + 1u == D.K ? 0u : slot + 1u;
-            }
-            if (!fin) {
-                for (int k = 0; k < 3; k++) D.sum32[(size_t)k * D.npix + px] = s[k];
-                continue;
-            }
-            float4* acc = (float4*)D.accum + px;
-            float inv = 1.0f / (float)D.spp;
-            float avg[3] = {s[0] * inv, s[1] * inv, s[2] * inv};
-            if (D.subframe > 1u) {
-                float4 prev = *acc;
-                float a = 1.0f / (float)D.subframe;
-                avg[0] = prev.x + a * (avg[0] - prev.x);
-                avg[1] = prev.y + a * (avg[1] - prev.y);
-                avg[2] = prev.z + a * (avg[2] - prev.z);
-            }
-            *acc = make_float4(avg[0], avg[1], avg[2], 1.0f);
-            for (int k = 0; k < 3; k++) oc[k] = avg[k];
+        const uint32_t e_lane = find_entry(plan->acc_prefix, n, j);
+        // one pass per distinct entry among the wave's lanes, with the entry
+        // and its request descriptor in SGPRs (waterfall bound: see regen)
+        bool todo = true;
+        for (uint32_t pass = 0; pass < 64u && todo; pass++) {
+            if (e_lane != __builtin_amdgcn_readfirstlane(e_lane)) continue;
+            todo = false;
+            const uint32_t e = uniform_copy(e_lane);
+            apply_pixel(req[plan->acc_req[e]], plan, e, j - plan->acc_prefix[e]);
         }
-        if (D.image) D.image[px] = make_color(oc[0], oc[1], oc[2]);
     }
 }
 
@@ -661,17 +685,17 @@ __device__ __forceinline__ void regen_rays(const ReqDesc* __restrict__ req, cons
         // wave straddles two requests' grants): the entry index is then
         // wave-uniform, so the plan entry and the request descriptor are read
         // by scalar loads into SGPRs instead of into every lane's VGPRs.
+        // Each pass retires at least the first remaining lane, so 64 passes
+        // cover a wave.  The bound also matters to the compiler: seen one
+        // thread at a time, an unbounded loop whose lane never matches would
+        // spin without side effects, which it may assume does not happen,
+        // and it then drops the comparison (it did so here, and every lane
+        // took the first lane's entry).
         bool todo = true;
-        while (todo) {
-            const uint32_t e0 = __builtin_amdgcn_readfirstlane(e_lane);
-            if (e_lane != e0) continue;
+        for (uint32_t pass = 0; pass < 64u && todo; pass++) {
+            if (e_lane != __builtin_amdgcn_readfirstlane(e_lane)) continue;
             todo = false;
-            // e0 == e_lane here, and the compiler would use the per-lane value
-            // for the loads; reading the first lane of a copy that the empty
-            // asm hides from it gives the same value as a wave-uniform one
-            uint32_t e_hid = e_lane;
-            asm volatile("" : "+v"(e_hid));
-            const uint32_t e = __builtin_amdgcn_readfirstlane(e_hid);
+            const uint32_t e = uniform_copy(e_lane);
             const uint32_t rq = plan->regen_req[e];
             const ReqDesc& D = req[rq];
             const unsigned long long base = plan->regen_base[e];
