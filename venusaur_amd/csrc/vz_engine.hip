@@ -502,7 +502,9 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     block_excl_scan_n<2>(gv, gp, gtot, s_scan);
     const uint32_t tg = gtot[0], short_pre = gp[1];
     // the rounding leftover (< number of short slices): one more ray each to
-    // the first short slices
+    // the first short slices.  The slices before this thread's got
+    // min(short_pre, left) of them, so its first slice starts there after
+    // the proportional shares: no further scan.
     const uint32_t left = totR - tg;
     uint32_t k = short_pre;
     for (uint32_t q = 0; q < SPT; q++) {
@@ -511,10 +513,7 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
             k++;
         }
     }
-    uint32_t my_final = 0;
-    for (uint32_t q = 0; q < SPT; q++) my_final += g[q];
-    uint32_t tfinal;
-    uint32_t run = block_excl_scan(my_final, s_scan, &tfinal);
+    uint32_t run = gp[0] + min(short_pre, left);
     for (uint32_t q = 0; q < SPT; q++) {
         const uint32_t sb = r * SPT + q;
         if (sb < NB) {
