@@ -237,12 +237,13 @@ NODE_BYTES = 64  # one 4-wide node: four quantised child boxes and the child wor
 LDS_WIDE_NODES = 256  # a scene rendered alone with at most this many 4-wide nodes is staged in LDS
 
 
-def survey_bytes(st, node_bytes=NODE_BYTES):
-    """The survey's algorithmic bytes per segment summed over a counting
-    launch: the node bytes fetched (a 4-wide node is 64 B, where the survey's
-    binary node pair was 32 B), 16 B per sphere tested, 16 B material per hit,
-    and 144 B of queue traffic (ray record written and read, hit record
-    written and read, path state read and written)."""
+def record_model_bytes(st, node_bytes=NODE_BYTES):
+    """Algorithmic bytes summed over a counting launch under a per-segment
+    model with separate ray, hit and path records: the node bytes fetched (a
+    4-wide node is 64 B; a binary node pair would be 32 B), 16 B per sphere
+    tested, 16 B material per hit, and 144 B of queue traffic per segment
+    (ray record written and read, hit record written and read, path state
+    read and written)."""
     return node_bytes * st.node_visits + 16 * st.sphere_tests + 16 * st.hits + 144 * st.segments
 
 
@@ -407,7 +408,7 @@ def run_c2(ctx, frames, seed=2024):
                            "algorithmic_bytes_per_launch": round(per_launch),
                            "algorithmic_gbs": round(per_launch / (avg_ms * 1e-3) / 1e9, 1),
                            "algorithmic_frac_of_hbm": round(per_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                           "survey_formula_frac_of_hbm": round(survey_bytes(cst) * frames / launches
+                           "record_model_frac_of_hbm": round(record_model_bytes(cst) * frames / launches
                                                                / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     out["step_kernel"].update(_traffic("step_kernel_traffic_c2.json", avg_ms))
     sq = _sq("box1M_1080p_64spp")
@@ -597,14 +598,14 @@ def main():
         # them out
         lds_nodes = cst.node_visits if lds_staged else 0
         hbm_alg = (bytes_step - NODE_BYTES * lds_nodes) * a.steps / launches
-        svy = survey_bytes(cst, nb) * a.steps / launches
+        svy = record_model_bytes(cst, nb) * a.steps / launches
         # with the nodes in LDS the algorithmic bytes are not HBM bytes: no
         # fraction of the HBM peak is given for them
         hbm = {"peak_gbs": HBM_PEAK_GBS, "node_source": "lds" if lds_staged else "hbm",
                "step_bytes_per_launch": round(per_launch_bytes), "step_bytes_gbs": round(achieved, 1),
                "step_bytes_frac": None if lds_staged else round(achieved / HBM_PEAK_GBS, 4),
-               "survey_formula_bytes_per_segment": round(survey_bytes(cst, nb) / seg_c, 1),
-               "survey_formula_frac": None if lds_staged else round(svy / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               "record_model_bytes_per_segment": round(record_model_bytes(cst, nb) / seg_c, 1),
+               "record_model_frac": None if lds_staged else round(svy / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                "node_bytes_from_lds_per_launch": round(NODE_BYTES * lds_nodes * a.steps / launches),
                "hbm_algorithmic_bytes_per_launch": round(hbm_alg),
                "hbm_algorithmic_frac": round(hbm_alg / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}

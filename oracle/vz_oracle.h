@@ -26,7 +26,8 @@
  *
  * Parity pinning (what checks each function, see tests/test_oracle.py):
  *   tea4/lcg/rnd ........ KATs compiled from the reference header itself
- *                         (oracle/_ref) and the survey's committed KAT table
+ *                         (oracle/_ref) and the committed KAT table
+ *                         (tests/golden/rng_kat.json)
  *   mt19937 ............. C++ standard check value (10000th output)
  *   book scene .......... committed sphere-list fixture + structural checks
  *   camera .............. closed forms (centre ray = W, corners W+-U/2+-V/2)
