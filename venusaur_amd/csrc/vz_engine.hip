@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
                     const uint32_t type = ldg(sc.leaf_info + tr.bleaf) >> 30;
                     const float4 mat = ldg(sc.mats + tr.bleaf);
                     pos = add(o, scl(d, tr.best));
-                    v3 n = scl(sub(pos, mk(s.x, s.y, s.z)), 1.0f / s.w);
+                    v3 n = scl(sub(pos, mk(s.x, s.y, s.z)), recip_rn(s.w));
                     const bool front = dot(d, n) < 0.0f;
                     if (!front) n = neg(n);
                     v3 att;

@@ -41,7 +41,7 @@ __device__ __forceinline__ v3 trace_path(const SceneView& sc, v3 o, v3 d, uint32
         uint32_t type = sc.leaf_info[r.leaf] >> 30;
         float4 mat = sc.mats[r.leaf];
         v3 p = add(o, scl(d, r.t));
-        v3 n = scl(sub(p, mk(s.x, s.y, s.z)), 1.0f / s.w);
+        v3 n = scl(sub(p, mk(s.x, s.y, s.z)), recip_rn(s.w));
         bool front = dot(d, n) < 0.0f;
         if (!front) n = neg(n);
         v3 nd, att;

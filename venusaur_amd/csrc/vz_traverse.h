@@ -104,7 +104,7 @@ struct Traversal {
         for (int k = 0; k < 3; k++) {
             float dk = dd[k];
             if (!(fabsf(dk) > 1e-20f)) dk = copysignf(1e-20f, dk);
-            inv[k] = 1.0f / dk;
+            inv[k] = recip_rn(dk);
             // lo <= hi and rounding is monotone, so (lo - o) * inv <= (hi - o) * inv
             // exactly when inv > 0: the near plane is lo for a positive
             // direction, hi for a negative one
