@@ -130,7 +130,7 @@ def test_c3_batch_64_scenes(ctx):
     say(f"  plain launches done: {plain_seg} segments")
     tunings = [None]
     t = api.Tuning()
-    t.queue_capacity, t.tile_w, t.tile_h, t.slice_split = 6 << 22, 8, 8, 4  # the autotuner's usual pick
+    t.queue_capacity, t.tile_w, t.tile_h, t.slice_split = 6 << 22, 8, 8, 4  # a 24M queue, square tiles, split 4
     tunings.append(t)
     for tu in tunings:
         for r in reqs:
