@@ -192,8 +192,9 @@ typedef struct vzTuning {
     uint32_t iters_per_graph; /* wavefront iterations captured per hipGraph (even) */
     uint32_t refill_lanes;    /* a wave fetches new rays once fewer than this many of its lanes
                                  still trace: 1 = only when all are done, 65 = after every
-                                 node step, 0 = the default for the scene's node placement
-                                 (33 for servers) */
+                                 node step, 0 = the default for the kernel variant: 9 for
+                                 one small scene with its nodes in LDS, 41 for one scene with
+                                 its nodes in memory, 33 for several scenes in one run */
     uint32_t slice_split;     /* queue slices per resident step workgroup: 1, 2, 4 or 8 (0 = 4 for
                                  batches and servers).  Slices of uneven cost (many scenes
                                  in one run) balance better when split; vzLaunch always uses 1 */
@@ -223,7 +224,7 @@ typedef struct vzServeConfig {
                                     they are not counted (vzAccelInfo.device_bytes gives theirs) */
     uint32_t max_active;         /* max requests in the device table (<= 1024) */
     vzTuning tuning;             /* 0 fields take the serving defaults: 48M-ray queue, 8x8 tiles,
-                                    queue split 4, refill at 33 lanes */
+                                    queue split 4 */
 } vzServeConfig;
 
 typedef struct vzRequestResult {

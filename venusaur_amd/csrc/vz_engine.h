@@ -45,11 +45,6 @@ constexpr uint32_t kBatchSplit = 4;
 // served 5648 Mray-segments/s at split 4 against 5518 at split 8; 32M: 5576
 // at 4, 5424 at 8, 5367 at 2; 24M: 5303 at 4, 5087 at 8.
 constexpr uint32_t kServeSplit = 4;
-// Servers go back for rays once fewer than 33 lanes of a wave still trace,
-// not the 41 the step kernel uses by default for nodes from memory (chosen
-// on one 1M-sphere scene): on the same 600 requests at 48M / 8x8 / split 4,
-// 5756 Mray-segments/s at 33, 5744 and 5702 at 25, 5669 and 5616 at 41.
-constexpr uint32_t kServeRefill = 33;
 constexpr uint32_t kDeadRay = 0xFFFFFFFFu;  // meta of a padding item: skipped
 // Scene table entries, allocated once when the engine is created, so that the
 // table's address never changes: captured hipGraphs bake it into their step

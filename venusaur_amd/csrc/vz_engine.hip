@@ -21,12 +21,17 @@ constexpr int kTbl = 128;       // LDS (request, pass) tally table entries
 constexpr uint32_t kLdsNodes = 256;  // a scene of at most this many 4-wide nodes is staged in LDS (16 KB)
 // A wave goes back for new rays once fewer than this many of its lanes still
 // trace (1 = only when all are done, 65 = after every node step).  Defaults per
-// node source, chosen by sweeping the 1080p book scene (LDS nodes) and
-// 1M-sphere box scenes (global nodes).
+// kernel variant, chosen by sweeping: one small scene with its nodes in LDS
+// (1080p book scene: 9), one scene with its nodes in memory (1M-sphere box
+// scene: 41), several scenes in one run (64-scene batch at 48M / 8x8 / split
+// 4: 5448 Mray-segments/s at 25, 5440 at 33, 5383 at 17, 5356 at 41; the
+// 2000-request stream's first 600 requests: 5756 at 33, 5744 and 5702 at 25,
+// 5669 and 5616 at 41).
 constexpr uint32_t kRefillLds = 9;
+constexpr uint32_t kRefillGlobal = 41;
+constexpr uint32_t kRefillMulti = 33;
 constexpr uint32_t kStrideMaxTiles = 24576;  // requests with fewer tiles visit them strided
 constexpr uint32_t kFlushDen = 2;  // queued sphere tests run once 1/kFlushDen of the tracing lanes hold some
-constexpr uint32_t kRefillGlobal = 41;
 constexpr int kRegenBlk = 256;
 constexpr uint32_t kRegenOver = 8;  // regen workgroups per queue slice: ~one pixel run per thread at 1080p
 constexpr int kBlk = 256;
@@ -928,7 +933,7 @@ static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool
                        e->q_o[p], e->q_d[p], e->q_t[p], e->q_o[n], e->q_d[n], e->q_t[n], cnt_cur, cnt_nxt,      \
                        e->d_bstats, e->R, refill)
     const uint32_t refill = e->cfg.refill_lanes ? std::min<uint32_t>(e->cfg.refill_lanes, 65u)
-                                                : (lds ? kRefillLds : kRefillGlobal);
+                                                : (lds ? kRefillLds : (single ? kRefillGlobal : kRefillMulti));
     if (count) {
         if (lds) VZ_STEP(true, true, true); else if (single) VZ_STEP(true, false, true); else VZ_STEP(true, false, false);
     } else {

@@ -220,8 +220,8 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     if (s->cfg.hbm_budget_bytes == 0) s->cfg.hbm_budget_bytes = ctx->hbm_bytes / 2;
     EngineConfig ec;
     // Serving defaults: a 48M-ray queue (4.8 GB for its two parities), 8x8
-    // camera tiles, the queue split 4 ways per resident step workgroup
-    // (kServeSplit) and refill at 33 lanes (kServeRefill).  On the first 600 requests of the 2000-request stream,
+    // camera tiles and the queue split 4 ways per resident step workgroup
+    // (kServeSplit).  On the first 600 requests of the 2000-request stream,
     // all in flight at once: 5648 Mray-segments/s against 5576 for 32M, 5303
     // for 24M and 5690 for 64M, all at split 4 (16x4 tiles at 24M: 5020).
     ec.queue_capacity = 12u << 22;
@@ -231,7 +231,7 @@ vzStatus vzServerCreate(vzContext ctx, const vzServeConfig* cfg, vzServer* out)
     if (s->cfg.tuning.tile_w) ec.tile_w = s->cfg.tuning.tile_w;
     if (s->cfg.tuning.tile_h) ec.tile_h = s->cfg.tuning.tile_h;
     if (s->cfg.tuning.iters_per_graph >= 2) ec.iters_per_graph = s->cfg.tuning.iters_per_graph;
-    ec.refill_lanes = s->cfg.tuning.refill_lanes ? std::min<uint32_t>(s->cfg.tuning.refill_lanes, 65u) : kServeRefill;
+    ec.refill_lanes = std::min<uint32_t>(s->cfg.tuning.refill_lanes, 65u);  // 0: the kernel variant's default
     vzStatus rs = engine_create(ctx, ec, &s->engine);
     if (rs == VZ_SUCCESS && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         rs = set_error(VZ_ERROR_HIP, "hipStreamCreateWithFlags failed");
