@@ -42,7 +42,6 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
-__device__ __forceinline__ uint32_t bcast(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {
@@ -644,7 +643,7 @@ __device__ __forceinline__ void apply_pixel(const ReqDesc& D, const Plan* __rest
 __device__ __forceinline__ uint32_t uniform_copy(uint32_t v)
 {
     asm volatile("" : "+v"(v));
-    return __builtin_amdgcn_readfirstlane(v);
+    return (uint32_t)__builtin_amdgcn_readfirstlane(v);
 }
 
 __device__ __forceinline__ void apply_passes(const ReqDesc* __restrict__ req, const Plan* __restrict__ plan,
@@ -657,7 +656,7 @@ __device__ __forceinline__ void apply_passes(const ReqDesc* __restrict__ req, co
         // and its request descriptor in SGPRs (waterfall bound: see regen)
         bool todo = true;
         for (uint32_t pass = 0; pass < 64u && todo; pass++) {
-            if (e_lane != __builtin_amdgcn_readfirstlane(e_lane)) continue;
+            if (e_lane != (uint32_t)__builtin_amdgcn_readfirstlane(e_lane)) continue;
             todo = false;
             const uint32_t e = uniform_copy(e_lane);
             apply_pixel(req[plan->acc_req[e]], plan, e, j - plan->acc_prefix[e]);
@@ -697,7 +696,7 @@ __device__ __forceinline__ void regen_rays(const ReqDesc* __restrict__ req, cons
         // took the first lane's entry).
         bool todo = true;
         for (uint32_t pass = 0; pass < 64u && todo; pass++) {
-            if (e_lane != __builtin_amdgcn_readfirstlane(e_lane)) continue;
+            if (e_lane != (uint32_t)__builtin_amdgcn_readfirstlane(e_lane)) continue;
             todo = false;
             const uint32_t e = uniform_copy(e_lane);
             const uint32_t rq = plan->regen_req[e];
