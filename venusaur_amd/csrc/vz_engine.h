@@ -153,6 +153,9 @@ struct Engine {
     hipGraphExec_t graph = nullptr;
     hipStream_t graph_stream = nullptr;
     int graph_key = -1;
+    // the control kernel runs 256 threads instead of 1024: at most 256
+    // request slots and 1024 queue slices (plain launches, small batches)
+    bool ctl_small = false;
     const SceneView* graph_scenes = nullptr;  // scene table the captured launches read
     // per-slot buffers, reused when large enough
     std::vector<DeviceBlock> slot_mem;

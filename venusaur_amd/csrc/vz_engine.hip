@@ -303,15 +303,16 @@ __global__ void __launch_bounds__(kStepBlk) __attribute__((amdgpu_waves_per_eu(6
 }
 
 // ---- control (one workgroup of 1024 threads) -----------------------------------
-// Exclusive scan over the 1024-thread control workgroup: an inclusive scan
-// inside each wave by shuffles, the 16 wave totals scanned by the first wave,
-// two barriers in all.  s needs 16 words.
+// Exclusive scan over the control workgroup of NW waves (16 or 4): an
+// inclusive scan inside each wave by shuffles, the NW wave totals scanned by
+// the first wave, two barriers in all.
 // N independent scans share one set of barriers and shuffle steps: out[k]
 // is the exclusive prefix of v[k], tot[k] the workgroup total.  s needs
 // 33 * N words.
-template <int N>
+template <int N, uint32_t NW = 16>
 __device__ void block_excl_scan_n(const uint32_t (&v)[N], uint32_t (&out)[N], uint32_t (&tot)[N], uint32_t* s)
 {
+    static_assert(NW >= 1 && NW <= 16, "the wave totals fit one 16-entry row");
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     uint32_t incl[N];
 #pragma unroll
@@ -330,15 +331,15 @@ __device__ void block_excl_scan_n(const uint32_t (&v)[N], uint32_t (&out)[N], ui
     if (wave == 0) {
 #pragma unroll
         for (int k = 0; k < N; k++) {
-            const uint32_t w = lane < 16u ? s[k * 33 + lane] : 0u;
+            const uint32_t w = lane < NW ? s[k * 33 + lane] : 0u;
             uint32_t wi = w;
 #pragma unroll
-            for (uint32_t off = 1; off < 16u; off <<= 1) {
+            for (uint32_t off = 1; off < NW; off <<= 1) {
                 const uint32_t x = __shfl_up(wi, off);
                 if (lane >= off) wi += x;
             }
-            if (lane < 16u) s[k * 33 + 16 + lane] = wi - w;  // exclusive prefix of the wave totals
-            if (lane == 15u) s[k * 33 + 32] = wi;
+            if (lane < NW) s[k * 33 + 16 + lane] = wi - w;  // exclusive prefix of the wave totals
+            if (lane == NW - 1u) s[k * 33 + 32] = wi;
         }
     }
     __syncthreads();
@@ -350,21 +351,25 @@ __device__ void block_excl_scan_n(const uint32_t (&v)[N], uint32_t (&out)[N], ui
     __syncthreads();  // s is reused by the next scan
 }
 
+template <uint32_t NW = 16>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s, uint32_t* total)
 {
     const uint32_t in[1] = {v};
     uint32_t out[1], tot[1];
-    block_excl_scan_n<1>(in, out, tot, s);
+    block_excl_scan_n<1, NW>(in, out, tot, s);
     *total = tot[0];
     return out[0];
 }
 
 
 
-// SPT: queue slices per control thread (NB <= 1024 * SPT); launched with
-// the smallest that covers the run's slices.
-template <uint32_t SPT>
-__global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__ req,
+// NT threads, one per request slot (n_slots <= NT); SPT: queue slices per
+// thread (NB <= NT * SPT).  1024 threads with the smallest SPT that covers
+// the run's slices, or 256 threads with SPT 4 for runs of at most 256
+// requests and 1024 slices, whose scans and barriers span 4 waves instead
+// of 16.
+template <uint32_t SPT, uint32_t NT = 1024>
+__global__ void __launch_bounds__(NT) k_wf_control(const ReqDesc* __restrict__ req,
                                                      ReqState* __restrict__ st, Counters* cnt,
                                                      Plan* __restrict__ plan,
                                                      uint32_t* __restrict__ cnt_next, uint32_t NB,
@@ -388,7 +393,7 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         my_free += fr[q];
     }
     uint32_t freeq;
-    (void)block_excl_scan(my_free, s_scan, &freeq);
+    (void)block_excl_scan<NT / 64>(my_free, s_scan, &freeq);
     // a request may start as many passes per iteration as its pass ring
     // allows; grants go to requests in slot (arrival) order, so the oldest
     // requests fill the queue first.  Fewer scenes in flight keep their BVHs
@@ -462,7 +467,7 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     const uint32_t sv[6] = {acc_cnt ? 1u : 0u, acc_cnt ? npix : 0u, grant ? 1u : 0u, grant, min(grant, npad),
                             still ? 1u : 0u};
     uint32_t sp[6], stot[6];
-    block_excl_scan_n<6>(sv, sp, stot, s_scan);
+    block_excl_scan_n<6, NT / 64>(sv, sp, stot, s_scan);
     const uint32_t posA = sp[0], pixA = sp[1], posR = sp[2], preR = sp[3], preRun = sp[4];
     const uint32_t nA = stot[0], totA = stot[1], nR = stot[2], totR = stot[3], totRuns = stot[4], nact = stot[5];
     if (acc_cnt) {
@@ -494,7 +499,7 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
         my_def += def[q];
     }
     uint32_t tdef;
-    (void)block_excl_scan(my_def, s_scan, &tdef);  // tdef >= totR
+    (void)block_excl_scan<NT / 64>(my_def, s_scan, &tdef);  // tdef >= totR
     uint32_t g[SPT], my_g = 0, my_short = 0;
     for (uint32_t q = 0; q < SPT; q++) {
         g[q] = tdef ? (uint32_t)((unsigned long long)totR * def[q] / tdef) : 0u;
@@ -503,7 +508,7 @@ __global__ void __launch_bounds__(1024) k_wf_control(const ReqDesc* __restrict__
     }
     const uint32_t gv[2] = {my_g, my_short};
     uint32_t gp[2], gtot[2];
-    block_excl_scan_n<2>(gv, gp, gtot, s_scan);
+    block_excl_scan_n<2, NT / 64>(gv, gp, gtot, s_scan);
     const uint32_t tg = gtot[0], short_pre = gp[1];
     // the rounding leftover (< number of short slices): one more ray each to
     // the first short slices.  The slices before this thread's got
@@ -940,7 +945,10 @@ static void enqueue_iteration(Engine* e, hipStream_t st, int p, bool count, bool
     }
 #undef VZ_STEP
     if (ev1) (void)hipEventRecord(ev1, st);
-    if (e->NB <= 4096u)
+    if (e->ctl_small)
+        hipLaunchKernelGGL((k_wf_control<4, 256>), dim3(1), dim3(256), 0, st, e->d_req, e->d_state, e->d_cnt,
+                           e->d_plan, cnt_nxt, e->NB, e->R);
+    else if (e->NB <= 4096u)
         hipLaunchKernelGGL(k_wf_control<4>, dim3(1), dim3(1024), 0, st, e->d_req, e->d_state, e->d_cnt, e->d_plan,
                            cnt_nxt, e->NB, e->R);
     else
@@ -956,7 +964,8 @@ static vzStatus ensure_graph(Engine* e, bool count, bool lds, bool single)
     // Every buffer the captured launches read is allocated once in
     // engine_create; what varies between runs (the queue partition and the
     // kernel variant) is in the key.
-    const int key = (count ? 1 : 0) | (lds ? 2 : 0) | (single ? 4 : 0) | (int)(e->split << 3);
+    const int key = (count ? 1 : 0) | (lds ? 2 : 0) | (single ? 4 : 0) | (int)(e->split << 3) |
+                    (e->ctl_small ? 1 << 7 : 0);
     if (e->graph && e->graph_key == key) return VZ_SUCCESS;
     if (e->graph) {
         (void)hipGraphExecDestroy(e->graph);
@@ -1126,6 +1135,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
         }
     }
     set_split(e, opt.slice_split);
+    e->ctl_small = n <= 256u && e->NB <= 1024u;
     VZ_TRY(upload_scenes(e, scenes, st));
     for (uint32_t i = 0; i < n; i++) VZ_TRY(setup_slot(e, i, scene_idx[i], items[i].params, i, st));
     VZ_HIP_TRY(hipMemcpyAsync(e->d_req, e->h_req, n * sizeof(ReqDesc), hipMemcpyHostToDevice, st));
@@ -1250,6 +1260,7 @@ vzStatus engine_run(Engine* e, const std::vector<RunItem>& items, hipStream_t st
 vzStatus engine_serve_init(Engine* e, hipStream_t st, uint32_t slice_split)
 {
     set_split(e, slice_split);
+    e->ctl_small = false;  // up to kMaxRequests request slots
     e->stream_alloc = true;
     e->slot_used.assign(kMaxRequests, 0);
     e->slot_scene.assign(kMaxRequests, 0);
