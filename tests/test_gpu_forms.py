@@ -110,7 +110,7 @@ def test_plain_batch_plain_on_one_context(ctx, scenes, oracle_results):
 def test_batch_at_the_request_table_limit(ctx, scenes):
     """1024 items, the request table's size, in one batch with the queue split
     4 and 1 ways: every accumulator equals a plain launch of the same item.
-    A 1025th item is refused with VZ_ERROR_CAPACITY.  The items have 48-98
+    A 1025th item is refused with VZ_ERROR_CAPACITY.  The items have 48-96
     pixels, so one wave of the finish kernel's apply and regen stages covers
     several requests: this exercises their per-request waterfall loops (a
     wave that took the first lane's request for all lanes left pixels
